@@ -1,0 +1,161 @@
+"""Benchmark: DOF-stage updates/s of the explicit DGLM RHS + LSRK4(5) step.
+
+Workload (one GPU): compressible Euler, isentropic vortex (gamma 1.4, strength 5,
+centre (5,5), free stream (1,1,1,0,1)) on [0,10]^2 x [0,10/3] with 32^3 hexes x
+6 tets = 196,608 tetrahedra, order N=4 (Np=35), LM flux, LSRK4(5) at CFL 0.4,
+FP64.  value = K * Np * 5 vars * 5 stages * steps / wall time, state resident
+in HBM (mesh, operators and initial state uploaded before the timed region).
+
+Prints one JSON line with the roofline of the dominant kernel (the fused
+stage kernel, timed with HIP events on the solver's stream) and a CPU baseline
+(the C restatement of the reference CPU path on this host's cores, bounded
+sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 (vector and matrix)
+
+
+def stage_bytes(K, Np, Nfp, nvar):
+    """Compulsory HBM bytes of one fused stage launch: u read, k read, k write,
+    u_out write (FP64), the per-element geometry record (25 doubles), packed
+    face connectivity (4 int32) and the element's own face wave speeds
+    (4 doubles)."""
+    return K * (32 * nvar * Np + 8 * 25 + 16 + 32)
+
+
+def stage_flops(K, Np, Nfp, nvar):
+    """Contraction flops of one stage, 2 K nvar Np (3 Np + 4 Nfp) (the model of
+    the reference's calc_ref, utils/src/python/gpu_profile.py:8-31)."""
+    return 2.0 * K * nvar * Np * (3 * Np + 4 * Nfp)
+
+
+def cpu_baseline(pb, u0, seconds_target=15.0):
+    """Time the oracle (C restatement, OpenMP) on a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(16, ncpu))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = O.Oracle(pb, "euler", cfl=0.4)
+    t0 = time.perf_counter()
+    o.run(u0, 1)
+    t1 = time.perf_counter()
+    per_step = t1 - t0
+    n = max(1, min(10, int(seconds_target / max(per_step, 1e-3))))
+    t0 = time.perf_counter()
+    o.run(u0, n)
+    wall = time.perf_counter() - t0
+    dof = pb.K * pb.Np * 5
+    return {"value": dof * 5 * n / wall, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} LSRK4(5) steps of the same workload (K={pb.K}, N={pb.N}) by oracle/dglm_oracle.c, "
+                      f"OpenMP {threads} threads, after 1 warm-up step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nhex", type=int, default=32)
+    ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU halo path is not built",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from exadglm_amd.problem import isentropic_vortex, uniform_problem, vortex_box
+    from exadglm_amd.solver import Solver
+
+    n1 = args.nhex + 1
+    pb = uniform_problem(args.order, n1, n1, n1, box=vortex_box(args.nhex))
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    K, Np, Nfp, nvar = pb.K, pb.Np, pb.Nfp, 5
+
+    s = Solver(pb, "euler", cfl=0.4, tmax=math.inf)
+    s.set_state(u0)
+    s.step(args.warmup)
+    s.sync()
+    s.set_timing(True)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    tm = s.timing()
+    t, dt, ts, _ = s.time()  # raises on a non-finite dt
+    u = s.get_state()
+    if not np.isfinite(u).all():
+        raise RuntimeError("state is not finite after the timed steps")
+    s.close()
+
+    dof_stage = K * Np * nvar * 5 * args.steps
+    value = dof_stage / wall
+    st_ms, st_n = tm["stage"]
+    st_avg_s = st_ms / st_n / 1e3
+    B = stage_bytes(K, Np, Nfp, nvar)
+    F = stage_flops(K, Np, Nfp, nvar)
+    gbs = B / st_avg_s / 1e9
+    tfl = F / st_avg_s / 1e12
+    t_hbm = B / (HBM_PEAK_GBS * 1e9)
+    t_fp = F / (FP64_PEAK_TFLOPS * 1e12)
+    if t_hbm >= t_fp:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    else:
+        roof = {"bound": "mfma", "achieved": tfl, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tfl / FP64_PEAK_TFLOPS}
+    roof["traffic"] = None
+    roof["kernel"] = "stage_simple_kernel<4,EULER,0>"
+    roof["avg_launch_us"] = st_avg_s * 1e6
+    roof["algorithmic_bytes_per_launch"] = B
+    roof["contraction_flops_per_launch"] = F
+    roof["other"] = ({"unit": "TFLOP/s", "achieved": tfl, "peak": FP64_PEAK_TFLOPS, "frac": tfl / FP64_PEAK_TFLOPS}
+                     if roof["bound"] == "hbm" else
+                     {"unit": "GB/s", "achieved": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS})
+    roof["binding_roof_us"] = max(t_hbm, t_fp) * 1e6
+    roof["frac_of_binding_roof"] = max(t_hbm, t_fp) / st_avg_s
+
+    out = {
+        "metric": "DOF-stage updates/s (FP64, Euler N=4 K=196608)",
+        "value": value,
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"euler_isentropic_vortex_N{args.order}_{args.nhex}cube_hexes_x6_K{K}_fp64",
+                   "K": K, "N": args.order, "Np": Np, "nvar": nvar, "cfl": 0.4, "precision": "fp64",
+                   "flux": "LM", "stepper": "LSRK4(5)"},
+        "roofline": roof,
+        "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in tm.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(pb, u0)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

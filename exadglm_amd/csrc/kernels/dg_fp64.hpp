@@ -1,0 +1,284 @@
+// FP64 DGLM kernels: wave speed, on-device dt, and the fused per-stage
+// RHS + LSRK kernel (volume flux, LM surface flux, stacked contraction with
+// [WDr|WDs|WDt|-LIFT], LSRK register update) -- one launch per stage.
+//
+// Math follows the reference CPU path:
+//   calc_maxv   dim3/src/cpp/euler_dglm.cpp:6-50,   advec_dglm.cpp:6-35
+//   calc_dt     euler_dglm.cpp:53-69
+//   update_tau  euler_dglm.cpp:72-84
+//   update_ub + calc_fluxLM  euler_dglm.cpp:87-183, advec_dglm.cpp:69-133
+//   calc_fluxes euler_dglm.cpp:186-251, advec_dglm.cpp:136-169
+//   update_rhs  euler_dglm.cpp:254-302
+//   lserk_stage euler_time.cpp:15-38
+#pragma once
+#include "dg_common.hpp"
+
+namespace dglm {
+
+enum { PHYS_ADV = 0, PHYS_EULER = 1 };
+
+template <int PHYS>
+struct NVar {
+  static constexpr int value = PHYS == PHYS_EULER ? 5 : 1;
+};
+
+// --------------------------------------------------------------------------
+// Pointwise physics
+// --------------------------------------------------------------------------
+
+// Euler fluxes F (x), G (y), H (z) of the five conserved variables.
+__device__ inline void euler_flux(const double u[5], double g, double F[5], double G[5], double H[5]) {
+  const double inv = 1.0 / u[0];
+  const double m2 = u[1] * u[1] + u[2] * u[2] + u[3] * u[3];
+  const double p = (g - 1.0) * (u[4] - 0.5 * m2 * inv);
+  const double vx = u[1] * inv, vy = u[2] * inv, vz = u[3] * inv;
+  const double ep = u[4] + p;
+  F[0] = u[1];
+  F[1] = u[1] * vx + p;
+  F[2] = u[2] * vx;
+  F[3] = u[3] * vx;
+  F[4] = ep * vx;
+  G[0] = u[2];
+  G[1] = u[1] * vy;
+  G[2] = u[2] * vy + p;
+  G[3] = u[3] * vy;
+  G[4] = ep * vy;
+  H[0] = u[3];
+  H[1] = u[1] * vz;
+  H[2] = u[2] * vz;
+  H[3] = u[3] * vz + p;
+  H[4] = ep * vz;
+}
+
+// --------------------------------------------------------------------------
+// Wave speed per face + block maximum of lambda*Fscale
+// --------------------------------------------------------------------------
+template <int N, int PHYS>
+__global__ __launch_bounds__(256) void wavespeed_kernel(int K, const double *__restrict__ u,
+                                                        const double *__restrict__ vel,
+                                                        const double *__restrict__ geo, const int *__restrict__ fmx,
+                                                        double *__restrict__ maxv, double *__restrict__ partial,
+                                                        double gamma) {
+  using O = Ord<N>;
+  constexpr int Np = O::Np, Nfp = O::Nfp;
+  const size_t KNp = (size_t)K * Np;
+  __shared__ double red[256];
+  const int fid = blockIdx.x * 256 + threadIdx.x;
+  double lfs = 0.0;
+  if (fid < K * 4) {
+    const int e = fid >> 2, f = fid & 3;
+    const int *fm = fmx + (f * 4 + 3) * Nfp;
+    double mv2 = 0.0, mc2 = 0.0;
+    for (int j = 0; j < Nfp; j++) {
+      const size_t idx = (size_t)e * Np + fm[j];
+      if (PHYS == PHYS_EULER) {
+        const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx],
+                     u5 = u[4 * KNp + idx];
+        const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        const double pres = (gamma - 1.0) * (u5 - 0.5 * u1 * v2);
+        const double c2 = fabs(gamma * pres / u1);
+        mv2 = v2 > mv2 ? v2 : mv2;
+        mc2 = c2 > mc2 ? c2 : mc2;
+      } else {
+        const double vx = vel[idx], vy = vel[KNp + idx], vz = vel[2 * KNp + idx];
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        mv2 = v2 > mv2 ? v2 : mv2;
+      }
+    }
+    const double lam = PHYS == PHYS_EULER ? sqrt(mv2) + sqrt(mc2) : sqrt(mv2);
+    maxv[fid] = lam;
+    lfs = lam * geo[(size_t)e * GEO_STRIDE + 9 + f * 4 + 3];
+  }
+  red[threadIdx.x] = lfs;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// dt = CFL/(2N+1) / max(lambda*Fscale), clipped so the run ends exactly at
+// tmax (euler_time.cpp:49-57).  One block.
+__global__ __launch_bounds__(1024) void finalize_dt_kernel(const double *__restrict__ partial, int n,
+                                                            DevScalars *s, double cfl_fac, double tmax,
+                                                            double *dt_hist, int hist_idx) {
+  __shared__ double red[1024];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, partial[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double maxfs = red[0];
+    const double t = s->t_next;
+    s->maxfs = maxfs;
+    if (t < tmax) {
+      double dt = cfl_fac * (1.0 / maxfs);
+      if (t + dt > tmax) dt = tmax - t;
+      s->t = t;
+      s->dt = dt;
+      s->t_next = t + dt;
+      s->tstep += 1;
+      s->active = 1;
+      if (!(dt > 0.0) || !isfinite(dt)) s->bad = 1;
+    } else {
+      s->active = 0;
+      s->dt = 0.0;
+    }
+    if (dt_hist) dt_hist[hist_idx] = s->active ? s->dt : 0.0;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Fused stage kernel, straightforward LDS-staged form.
+//   MODE 0: k = a k + dt rhs; u_out = u_in + b k
+//   MODE 1: rhs only
+// --------------------------------------------------------------------------
+struct StageArgs {
+  int K;
+  const double *u_in;
+  double *u_out;
+  double *k;
+  double *rhs;
+  const double *vel;
+  const double *geo;
+  const int *conn;
+  const int *fmx;
+  const double *op;   // [KD][Np]: WDr^T, WDs^T, WDt^T, -LIFT^T
+  const double *maxv;
+  const DevScalars *scal;
+  double rka, rkb;
+  double gamma, tau_scale;
+};
+
+template <int N, int PHYS>
+constexpr int eb_simple() {
+  using O = Ord<N>;
+  constexpr int NV = NVar<PHYS>::value;
+  constexpr int bytes =
+      8 * (NV * (O::Np + O::KD) + (PHYS == PHYS_ADV ? 3 * O::Np : 0) + GEO_STRIDE + 4) + 16;
+  constexpr int eb = 56000 / bytes;
+  return eb < 1 ? 1 : (eb > 64 ? 64 : eb);
+}
+
+template <int N, int PHYS, int MODE>
+__global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
+  using O = Ord<N>;
+  constexpr int Np = O::Np, Nfp = O::Nfp, NFF = O::NFF, KD = O::KD;
+  constexpr int NV = NVar<PHYS>::value;
+  constexpr int EB = eb_simple<N, PHYS>();
+  constexpr int NVEL = PHYS == PHYS_ADV ? 3 : 1;
+  __shared__ double s_u[NV][EB * Np];
+  __shared__ double s_f[NV][EB * KD];
+  __shared__ double s_vel[NVEL][PHYS == PHYS_ADV ? EB * Np : 1];
+  __shared__ double s_geo[EB * GEO_STRIDE];
+  __shared__ double s_tau[EB * 4];
+  __shared__ int s_conn[EB * 4];
+  __shared__ int s_fmx[16 * Nfp];
+
+  const int K = a.K;
+  const size_t KNp = (size_t)K * Np;
+  const int e0 = blockIdx.x * EB;
+  const int ne = min(EB, K - e0);
+  const int tid = threadIdx.x;
+
+  if (MODE == 0 && !a.scal->active) {
+    // A step past tmax: keep the state (the caller's buffer swap still happens).
+    for (int v = 0; v < NV; v++)
+      for (int i = tid; i < ne * Np; i += 256) a.u_out[v * KNp + (size_t)e0 * Np + i] = a.u_in[v * KNp + (size_t)e0 * Np + i];
+    return;
+  }
+
+  for (int i = tid; i < 16 * Nfp; i += 256) s_fmx[i] = a.fmx[i];
+  for (int i = tid; i < ne * GEO_STRIDE; i += 256) s_geo[i] = a.geo[(size_t)e0 * GEO_STRIDE + i];
+  for (int i = tid; i < ne * 4; i += 256) {
+    const int c = a.conn[(size_t)e0 * 4 + i];
+    s_conn[i] = c;
+    const int nf = conn_nbr(c) * 4 + conn_face(c);
+    s_tau[i] = 0.25 * (a.maxv[(size_t)e0 * 4 + i] + a.maxv[nf]) * a.tau_scale;
+  }
+  for (int v = 0; v < NV; v++)
+    for (int i = tid; i < ne * Np; i += 256) s_u[v][i] = a.u_in[v * KNp + (size_t)e0 * Np + i];
+  if (PHYS == PHYS_ADV)
+    for (int d = 0; d < 3; d++)
+      for (int i = tid; i < ne * Np; i += 256) s_vel[d][i] = a.vel[d * KNp + (size_t)e0 * Np + i];
+  __syncthreads();
+
+  // Volume fluxes: fR = rx F + ry G + rz H, fS, fT likewise.
+  for (int idx = tid; idx < ne * Np; idx += 256) {
+    const int e = idx / Np, i = idx - e * Np;
+    const double *g = &s_geo[e * GEO_STRIDE];
+    if (PHYS == PHYS_EULER) {
+      double u[5], F[5], G[5], H[5];
+      for (int v = 0; v < 5; v++) u[v] = s_u[v][idx];
+      euler_flux(u, a.gamma, F, G, H);
+      for (int v = 0; v < 5; v++) {
+        double *fo = &s_f[v][e * KD];
+        fo[i] = g[0] * F[v] + g[1] * G[v] + g[2] * H[v];
+        fo[Np + i] = g[3] * F[v] + g[4] * G[v] + g[5] * H[v];
+        fo[2 * Np + i] = g[6] * F[v] + g[7] * G[v] + g[8] * H[v];
+      }
+    } else {
+      const double uu = s_u[0][idx];
+      const double F = s_vel[0][idx] * uu, G = s_vel[1][idx] * uu, H = s_vel[2][idx] * uu;
+      double *fo = &s_f[0][e * KD];
+      fo[i] = g[0] * F + g[1] * G + g[2] * H;
+      fo[Np + i] = g[3] * F + g[4] * G + g[5] * H;
+      fo[2 * Np + i] = g[6] * F + g[7] * G + g[8] * H;
+    }
+  }
+
+  // LM surface flux (n.F(ub) - tau (ub - u-)) Fscale with ub = (u- + u+)/2.
+  for (int idx = tid; idx < ne * NFF; idx += 256) {
+    const int e = idx / NFF, m = idx - e * NFF;
+    const int f = m / Nfp, j = m - f * Nfp;
+    const int c = s_conn[e * 4 + f];
+    const size_t pidx = (size_t)conn_nbr(c) * Np + s_fmx[(conn_face(c) * 4 + conn_code(c)) * Nfp + j];
+    const int mloc = e * Np + s_fmx[(f * 4 + 3) * Nfp + j];
+    const double *g = &s_geo[e * GEO_STRIDE + 9 + f * 4];
+    const double nx = g[0], ny = g[1], nz = g[2], fs = g[3];
+    const double tau = s_tau[e * 4 + f];
+    if (PHYS == PHYS_EULER) {
+      double uM[5], ub[5], F[5], G[5], H[5];
+      for (int v = 0; v < 5; v++) {
+        uM[v] = s_u[v][mloc];
+        ub[v] = 0.5 * (uM[v] + a.u_in[v * KNp + pidx]);
+      }
+      euler_flux(ub, a.gamma, F, G, H);
+      for (int v = 0; v < 5; v++)
+        s_f[v][e * KD + 3 * Np + m] = ((nx * F[v] + ny * G[v] + nz * H[v]) - tau * (ub[v] - uM[v])) * fs;
+    } else {
+      const double uM = s_u[0][mloc];
+      const double ub = 0.5 * (uM + a.u_in[pidx]);
+      const double nf = nx * (s_vel[0][mloc] * ub) + ny * (s_vel[1][mloc] * ub) + nz * (s_vel[2][mloc] * ub);
+      s_f[0][e * KD + 3 * Np + m] = (nf - tau * (ub - uM)) * fs;
+    }
+  }
+  __syncthreads();
+
+  // Contraction rhs_i = sum_k op[k][i] f[k], then the LSRK register update.
+  const double dt = MODE == 0 ? a.scal->dt : 0.0;
+  for (int o = tid; o < NV * ne * Np; o += 256) {
+    const int v = o / (ne * Np), r = o - v * (ne * Np);
+    const int e = r / Np, i = r - e * Np;
+    const double *F = &s_f[v][e * KD];
+    double acc = 0.0;
+#pragma unroll 5
+    for (int kk = 0; kk < KD; kk++) acc += a.op[kk * Np + i] * F[kk];
+    const size_t gi = v * KNp + (size_t)e0 * Np + r;
+    if (MODE == 1) {
+      a.rhs[gi] = acc;
+    } else {
+      const double kn = a.rka * a.k[gi] + dt * acc;
+      a.k[gi] = kn;
+      a.u_out[gi] = s_u[v][r] + a.rkb * kn;
+    }
+  }
+}
+
+}  // namespace dglm

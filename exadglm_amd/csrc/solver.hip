@@ -1,0 +1,756 @@
+// Solver context and C ABI of libexadglm_hip.so (see include/exadglm.h).
+//
+// One context = one single-partition problem resident in HBM:
+//   state u[2] (ping-pong between stages), LSRK register k, per-element
+//   geometry records, packed face connectivity, the stacked operator, per-face
+//   wave speeds and the device-side step scalars (t, dt, tstep).
+// Every launch goes to the context's stream; nothing synchronises inside a
+// step (the reference blocks on a D2H copy of dt each step,
+// euler_nopad_dglm.cu:79-96).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/exadglm.h"
+#include "host/mesh.h"
+#include "host/refelem.h"
+#include "kernels/dg_fp64.hpp"
+
+#define DGLM_NTIMERS 3  // wave speed, dt finalize, stage
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return fail(DGLM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+// Low-storage RK4(5) of Carpenter & Kennedy (1994), 2N-storage form
+// (coefficients as in dim3/src/cpp/euler_data.cpp:41-60).
+const double RKA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                       -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+const double RKB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                       1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                       2277821191437.0 / 14882151754819.0};
+}  // namespace
+
+using namespace dglm;
+
+struct dglm_ctx {
+  dglm_params p{};
+  int K = 0, Np = 0, Nfp = 0, NFF = 0, KD = 0, nvar = 0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  double *u[2] = {nullptr, nullptr};
+  double *k = nullptr, *rhs = nullptr, *vel = nullptr, *geo = nullptr, *op = nullptr, *maxv = nullptr,
+         *partial = nullptr, *dt_hist = nullptr;
+  int dt_hist_cap = 0;
+  int *conn = nullptr, *fmx = nullptr;
+  DevScalars *scal = nullptr;
+  int cur = 0;
+  int nblk_ws = 0;
+  std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
+  // Optional per-kernel timing with HIP events on the context's stream.
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_class;  // kernel class of each recorded (start, stop) pair
+  double t_ms[DGLM_NTIMERS] = {0};
+  long t_cnt[DGLM_NTIMERS] = {0};
+};
+
+extern "C" const char *dglm_last_error(void) { return g_err.c_str(); }
+extern "C" int dglm_version(void) { return 100; }
+
+// ---------------------------------------------------------------------------
+// Reference-element tables
+// ---------------------------------------------------------------------------
+extern "C" int dglm_ref_sizes(int N, int *Np, int *Nfp) {
+  if (N < 1 || N > 14) return fail(DGLM_ERR_ARG, "N must be in [1, 14]");
+  if (Np) *Np = tet_np(N);
+  if (Nfp) *Nfp = tri_np(N);
+  return DGLM_OK;
+}
+
+extern "C" int dglm_ref_operators(int N, const dglm_ref_tables *out) {
+  if (!out) return fail(DGLM_ERR_ARG, "dglm_ref_operators: null output table");
+  try {
+    RefOps r;
+    reference_operators(N, r);
+    auto cp = [](const std::vector<double> &v, double *d) {
+      if (d) memcpy(d, v.data(), v.size() * sizeof(double));
+    };
+    cp(r.xyz, out->xyz);
+    cp(r.rst, out->rst);
+    cp(r.V, out->V);
+    cp(r.M, out->M);
+    cp(r.invM, out->invM);
+    cp(r.Dr, out->Dr);
+    cp(r.Ds, out->Ds);
+    cp(r.Dt, out->Dt);
+    cp(r.WDr, out->WDr);
+    cp(r.WDs, out->WDs);
+    cp(r.WDt, out->WDt);
+    cp(r.Emat, out->Emat);
+    cp(r.LIFT, out->LIFT);
+    if (out->Fmask) memcpy(out->Fmask, r.Fmask.data(), r.Fmask.size() * sizeof(int));
+    if (out->FmaskP) memcpy(out->FmaskP, r.FmaskP.data(), r.FmaskP.size() * sizeof(int));
+  } catch (std::exception &ex) {
+    return fail(DGLM_ERR_ARG, ex.what());
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_interp_operators(int N, int high_N, double *high_rst, double *interpM, double *high_M) {
+  try {
+    std::vector<double> a, b, c;
+    interp_operators(N, high_N, a, b, c);
+    if (high_rst) memcpy(high_rst, a.data(), a.size() * sizeof(double));
+    if (interpM) memcpy(interpM, b.data(), b.size() * sizeof(double));
+    if (high_M) memcpy(high_M, c.data(), c.size() * sizeof(double));
+  } catch (std::exception &ex) {
+    return fail(DGLM_ERR_ARG, ex.what());
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_gll_nodes(int N, double *x) {
+  if (N < 1 || N > 30 || !x) return fail(DGLM_ERR_ARG, "dglm_gll_nodes: bad arguments");
+  std::vector<long double> g = gll_nodes(N);
+  for (int i = 0; i <= N; i++) x[i] = (double)g[i];
+  return DGLM_OK;
+}
+
+extern "C" int dglm_jacobi(int n, double a, double b, int npts, const double *x, double *P, double *dP) {
+  if (n < 0 || npts < 0 || !x) return fail(DGLM_ERR_ARG, "dglm_jacobi: bad arguments");
+  for (int i = 0; i < npts; i++) {
+    if (P) P[i] = (double)jacobi_value(n, a, b, x[i]);
+    if (dP) dP[i] = (double)jacobi_grad(n, a, b, x[i]);
+  }
+  return DGLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mesh
+// ---------------------------------------------------------------------------
+extern "C" int dglm_uniform_mesh(int nx, int ny, int nz, const double *box, const int *phy_tags, double *VXYZ,
+                                 int *EtoV, int *EtoE, int *EtoF, int *EtoR, int *EtoB) {
+  if (!box || !phy_tags || !VXYZ || !EtoV || !EtoE || !EtoF || !EtoR || !EtoB)
+    return fail(DGLM_ERR_ARG, "dglm_uniform_mesh: null argument");
+  if (nx < 2 || ny < 2 || nz < 2) return fail(DGLM_ERR_ARG, "dglm_uniform_mesh: need nx, ny, nz >= 2");
+  try {
+    uniform_vertices(nx, ny, nz, box[0], box[1], box[2], box[3], box[4], box[5], VXYZ);
+    uniform_connectivity(nx, ny, nz, phy_tags, EtoV, EtoE, EtoF, EtoR, EtoB);
+  } catch (std::exception &ex) {
+    return fail(DGLM_ERR_ARG, ex.what());
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_mesh_points(int K, int Np, const int *EtoV, const double *VXYZ, const double *rst, double *PX,
+                                double *PY, double *PZ) {
+  if (K < 0 || Np <= 0 || !EtoV || !VXYZ || !rst || !PX || !PY || !PZ)
+    return fail(DGLM_ERR_ARG, "dglm_mesh_points: bad arguments");
+  element_points(K, Np, EtoV, VXYZ, rst, PX, PY, PZ);
+  return DGLM_OK;
+}
+
+extern "C" int dglm_mesh_metrics(int K, const int *EtoV, const double *VXYZ, double *J, double *metrics) {
+  if (K < 0 || !EtoV || !VXYZ || !J || !metrics) return fail(DGLM_ERR_ARG, "dglm_mesh_metrics: bad arguments");
+  double *m[9];
+  for (int i = 0; i < 9; i++) m[i] = metrics + (size_t)i * K;
+  element_metrics(K, EtoV, VXYZ, J, m);
+  return DGLM_OK;
+}
+
+extern "C" int dglm_mesh_normals(int K, const double *J, const double *metrics, double *nx, double *ny, double *nz,
+                                 double *sJ, double *Fscale) {
+  if (K < 0 || !J || !metrics || !nx || !ny || !nz || !sJ || !Fscale)
+    return fail(DGLM_ERR_ARG, "dglm_mesh_normals: bad arguments");
+  const double *m[9];
+  for (int i = 0; i < 9; i++) m[i] = metrics + (size_t)i * K;
+  face_normals(K, J, (double *const *)m, nx, ny, nz, sJ, Fscale);
+  return DGLM_OK;
+}
+
+extern "C" int dglm_mesh_vmaps(int K, int N, const int *EtoE, const int *EtoF, const int *EtoR, int *vmapM,
+                               int *vmapP, int *vmapF) {
+  if (K < 0 || !EtoE || !EtoF || !EtoR || !vmapM || !vmapP || !vmapF)
+    return fail(DGLM_ERR_ARG, "dglm_mesh_vmaps: bad arguments");
+  try {
+    RefOps r;
+    reference_operators(N, r);
+    face_vmaps(K, r.Np, r.Nfp, EtoE, EtoF, EtoR, r.Fmask.data(), r.FmaskP.data(), vmapM, vmapP, vmapF);
+  } catch (std::exception &ex) {
+    return fail(DGLM_ERR_ARG, ex.what());
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_mesh_spatial_h(int K, const int *EtoV, const double *VXYZ, double *h) {
+  if (K < 0 || !EtoV || !VXYZ || !h) return fail(DGLM_ERR_ARG, "dglm_mesh_spatial_h: bad arguments");
+  spatial_h(K, EtoV, VXYZ, h);
+  return DGLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int N, int PHYS>
+void launch_wavespeed(dglm_ctx *c) {
+  hipLaunchKernelGGL((wavespeed_kernel<N, PHYS>), dim3(c->nblk_ws), dim3(256), 0, c->stream, c->K, c->u[c->cur],
+                     c->vel, c->geo, c->fmx, c->maxv, c->partial, c->p.gamma);
+}
+
+template <int N, int PHYS, int MODE>
+void launch_stage(dglm_ctx *c, const StageArgs &a) {
+  constexpr int EB = eb_simple<N, PHYS>();
+  int nblk = (c->K + EB - 1) / EB;
+  hipLaunchKernelGGL((stage_simple_kernel<N, PHYS, MODE>), dim3(nblk), dim3(256), 0, c->stream, a);
+}
+
+typedef void (*ws_fn)(dglm_ctx *);
+typedef void (*st_fn)(dglm_ctx *, const StageArgs &);
+
+template <int PHYS>
+bool pick(int N, ws_fn &ws, st_fn &st0, st_fn &st1) {
+  switch (N) {
+#define DGLM_CASE(n)                                   \
+  case n:                                              \
+    ws = launch_wavespeed<n, PHYS>;                    \
+    st0 = launch_stage<n, PHYS, 0>;                    \
+    st1 = launch_stage<n, PHYS, 1>;                    \
+    return true;
+    DGLM_CASE(1)
+    DGLM_CASE(2)
+    DGLM_CASE(3)
+    DGLM_CASE(4)
+    DGLM_CASE(5)
+    DGLM_CASE(6)
+#undef DGLM_CASE
+    default:
+      return false;
+  }
+}
+
+bool dispatch(const dglm_ctx *c, ws_fn &ws, st_fn &st0, st_fn &st1) {
+  return c->p.physics == DGLM_EULER ? pick<PHYS_EULER>(c->p.N, ws, st0, st1)
+                                    : pick<PHYS_ADV>(c->p.N, ws, st0, st1);
+}
+
+StageArgs stage_args(dglm_ctx *c, int s) {
+  StageArgs a;
+  a.K = c->K;
+  a.u_in = c->u[c->cur];
+  a.u_out = c->u[c->cur ^ 1];
+  a.k = c->k;
+  a.rhs = c->rhs;
+  a.vel = c->vel;
+  a.geo = c->geo;
+  a.conn = c->conn;
+  a.fmx = c->fmx;
+  a.op = c->op;
+  a.maxv = c->maxv;
+  a.scal = c->scal;
+  a.rka = s >= 0 ? RKA[s] : 0.0;
+  a.rkb = s >= 0 ? RKB[s] : 0.0;
+  a.gamma = c->p.gamma;
+  a.tau_scale = c->p.tau_scale;
+  return a;
+}
+
+template <typename T>
+int dev_alloc(T **p, size_t n, const char *what) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T) + 16);
+  if (e != hipSuccess) return fail(DGLM_ERR_HIP, std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
+  return DGLM_OK;
+}
+
+void free_ctx(dglm_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
+                  c->conn, c->fmx, c->scal};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// Record the start event of a timed launch; returns the pair index or -1.
+int timer_start(dglm_ctx *c, int cls) {
+  if (!c->timing) return -1;
+  size_t idx = c->ev_class.size();
+  while (c->ev_pool.size() < 2 * (idx + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    c->ev_pool.push_back(e);
+  }
+  c->ev_class.push_back(cls);
+  hipEventRecord(c->ev_pool[2 * idx], c->stream);
+  return (int)idx;
+}
+
+void timer_stop(dglm_ctx *c, int idx) {
+  if (idx >= 0) hipEventRecord(c->ev_pool[2 * idx + 1], c->stream);
+}
+
+int timer_flush(dglm_ctx *c) {
+  if (c->ev_class.empty()) return DGLM_OK;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(DGLM_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  for (size_t i = 0; i < c->ev_class.size(); i++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_pool[2 * i], c->ev_pool[2 * i + 1]) == hipSuccess) {
+      c->t_ms[c->ev_class[i]] += ms;
+      c->t_cnt[c->ev_class[i]] += 1;
+    }
+  }
+  c->ev_class.clear();
+  return DGLM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **out) {
+  if (!p || !m || !out) return fail(DGLM_ERR_ARG, "dglm_create: null argument");
+  *out = nullptr;
+  if (p->physics != DGLM_ADVECTION && p->physics != DGLM_EULER)
+    return fail(DGLM_ERR_ARG, "dglm_create: physics must be DGLM_ADVECTION or DGLM_EULER");
+  if (p->precision != DGLM_FP64) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: precision mode not built");
+  if (p->flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: only the LM flux is built");
+  if (p->N < 1 || p->N > 6) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: N must be in [1, 6]");
+  if (m->K <= 0) return fail(DGLM_ERR_ARG, "dglm_create: K must be positive");
+  if (!(p->cfl > 0) || !(p->tau_scale >= 0) || !(p->gamma > 1.0) || std::isnan(p->tmax))
+    return fail(DGLM_ERR_ARG, "dglm_create: invalid cfl / gamma / tau_scale / tmax");
+  const double *req[] = {m->WDr, m->WDs, m->WDt, m->LIFT, m->Fscale, m->rx, m->ry, m->rz, m->sx,
+                         m->sy,  m->sz,  m->tx,  m->ty,   m->tz,     m->nx, m->ny, m->nz};
+  for (const double *q : req)
+    if (!q) return fail(DGLM_ERR_ARG, "dglm_create: missing mesh / operator array");
+  if (!m->vmapP || !m->vmapF || !m->Fmask) return fail(DGLM_ERR_ARG, "dglm_create: missing index array");
+
+  RefOps ref;
+  try {
+    reference_operators(p->N, ref);
+  } catch (std::exception &ex) {
+    return fail(DGLM_ERR_ARG, ex.what());
+  }
+  const int K = m->K, Np = ref.Np, Nfp = ref.Nfp, NFF = 4 * Nfp, KD = 3 * Np + NFF;
+  for (int i = 0; i < 4 * Nfp; i++)
+    if (m->Fmask[i] != ref.Fmask[i]) return fail(DGLM_ERR_ARG, "dglm_create: Fmask does not match order N");
+
+  // Face-node table: code 0..2 = FmaskP[f][rot], code 3 = Fmask[f].
+  std::vector<int> fmx(16 * Nfp);
+  for (int f = 0; f < 4; f++)
+    for (int cde = 0; cde < 4; cde++)
+      for (int j = 0; j < Nfp; j++)
+        fmx[(f * 4 + cde) * Nfp + j] = cde < 3 ? ref.FmaskP[(f * 3 + cde) * Nfp + j] : ref.Fmask[f * Nfp + j];
+
+  // Packed connectivity from (vmapF, vmapP): find the neighbour face and the
+  // node permutation that reproduces vmapP exactly.
+  std::vector<int> conn((size_t)K * 4);
+  for (int e = 0; e < K; e++)
+    for (int f = 0; f < 4; f++) {
+      const int o = e * 4 + f;
+      const int vf = m->vmapF[o];
+      if (vf < 0 || vf >= 4 * K) return fail(DGLM_ERR_ARG, "dglm_create: vmapF out of range");
+      const int nb = vf / 4, nf = vf % 4;
+      int found = -1;
+      for (int cde = 0; cde < 4 && found < 0; cde++) {
+        bool ok = true;
+        for (int j = 0; j < Nfp && ok; j++) ok = m->vmapP[(size_t)o * Nfp + j] == nb * Np + fmx[(nf * 4 + cde) * Nfp + j];
+        if (ok) found = cde;
+      }
+      if (found < 0)
+        return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: vmapP of face " + std::to_string(o) +
+                                              " is not a rotation of its neighbour face");
+      if (m->EtoB && m->EtoB[o] == 1 && found != 3)
+        return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: Dirichlet faces (EtoB == 1) are not implemented");
+      conn[o] = (nb << 4) | (nf << 2) | found;
+    }
+
+  // Geometry records.
+  std::vector<double> geo((size_t)K * GEO_STRIDE);
+  const double *mt[9] = {m->rx, m->ry, m->rz, m->sx, m->sy, m->sz, m->tx, m->ty, m->tz};
+  for (int e = 0; e < K; e++) {
+    double *g = &geo[(size_t)e * GEO_STRIDE];
+    for (int q = 0; q < 9; q++) g[q] = mt[q][e];
+    for (int f = 0; f < 4; f++) {
+      g[9 + f * 4 + 0] = m->nx[e * 4 + f];
+      g[9 + f * 4 + 1] = m->ny[e * 4 + f];
+      g[9 + f * 4 + 2] = m->nz[e * 4 + f];
+      g[9 + f * 4 + 3] = m->Fscale[e * 4 + f];
+    }
+  }
+
+  // Stacked operator, column-major [KD][Np]: op[k][i].
+  std::vector<double> op((size_t)KD * Np);
+  auto at = [&](const double *A, int rows, int cols, int i, int j) {
+    return m->matrix_order == 0 ? A[(size_t)i * cols + j] : A[(size_t)j * rows + i];
+  };
+  const double *WD[3] = {m->WDr, m->WDs, m->WDt};
+  for (int d = 0; d < 3; d++)
+    for (int j = 0; j < Np; j++)
+      for (int i = 0; i < Np; i++) op[(size_t)(d * Np + j) * Np + i] = at(WD[d], Np, Np, i, j);
+  for (int q = 0; q < NFF; q++)
+    for (int i = 0; i < Np; i++) op[(size_t)(3 * Np + q) * Np + i] = -at(m->LIFT, Np, NFF, i, q);
+
+  dglm_ctx *c = new dglm_ctx();
+  c->p = *p;
+  c->K = K;
+  c->Np = Np;
+  c->Nfp = Nfp;
+  c->NFF = NFF;
+  c->KD = KD;
+  c->nvar = p->physics == DGLM_EULER ? 5 : 1;
+  c->dev = p->device;
+  c->nblk_ws = (K * 4 + 255) / 256;
+  c->h_conn = conn;
+  int rc;
+  hipError_t he = hipSetDevice(c->dev);
+  if (he != hipSuccess) {
+    delete c;
+    return fail(DGLM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(he));
+  }
+  const size_t nst = (size_t)c->nvar * K * Np;
+  if ((rc = dev_alloc(&c->u[0], nst, "u0")) || (rc = dev_alloc(&c->u[1], nst, "u1")) ||
+      (rc = dev_alloc(&c->k, nst, "k")) || (rc = dev_alloc(&c->geo, geo.size(), "geo")) ||
+      (rc = dev_alloc(&c->op, op.size(), "op")) || (rc = dev_alloc(&c->maxv, (size_t)K * 4, "maxv")) ||
+      (rc = dev_alloc(&c->partial, (size_t)c->nblk_ws, "partial")) ||
+      (rc = dev_alloc(&c->conn, conn.size(), "conn")) || (rc = dev_alloc(&c->fmx, fmx.size(), "fmx")) ||
+      (rc = dev_alloc(&c->scal, 1, "scalars"))) {
+    free_ctx(c);
+    return rc;
+  }
+  if (c->nvar == 1 && (rc = dev_alloc(&c->vel, (size_t)3 * K * Np, "velocity"))) {
+    free_ctx(c);
+    return rc;
+  }
+  he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    free_ctx(c);
+    return fail(DGLM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+  }
+  DevScalars s0{};
+  s0.active = 1;
+  hipError_t errs[] = {
+      hipMemcpy(c->geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice),
+      hipMemcpy(c->op, op.data(), op.size() * sizeof(double), hipMemcpyHostToDevice),
+      hipMemcpy(c->conn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice),
+      hipMemcpy(c->fmx, fmx.data(), fmx.size() * sizeof(int), hipMemcpyHostToDevice),
+      hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
+      hipMemset(c->u[0], 0, nst * sizeof(double)),
+      hipMemset(c->u[1], 0, nst * sizeof(double)),
+      hipMemset(c->k, 0, nst * sizeof(double)),
+  };
+  for (hipError_t e : errs)
+    if (e != hipSuccess) {
+      free_ctx(c);
+      return fail(DGLM_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+  if (c->vel) {
+    // Default advection velocity (1,1,1), as advec_init.cpp:17-19.
+    std::vector<double> one((size_t)3 * K * Np, 1.0);
+    if (hipMemcpy(c->vel, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      free_ctx(c);
+      return fail(DGLM_ERR_HIP, "upload velocity failed");
+    }
+  }
+  *out = c;
+  return DGLM_OK;
+}
+
+namespace {
+template <typename T>
+int read_file(const std::string &path, std::vector<T> &v, size_t n) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f.is_open()) return fail(DGLM_ERR_IO, "cannot open " + path);
+  v.resize(n);
+  f.read((char *)v.data(), n * sizeof(T));
+  if ((size_t)f.gcount() != n * sizeof(T))
+    return fail(DGLM_ERR_IO, path + ": expected " + std::to_string(n * sizeof(T)) + " bytes, got " +
+                                 std::to_string(f.gcount()));
+  return DGLM_OK;
+}
+}  // namespace
+
+extern "C" int dglm_create_from_dir(const dglm_params *p, const char *datain_dir, int matrix_order, dglm_ctx **out) {
+  if (!p || !datain_dir || !out) return fail(DGLM_ERR_ARG, "dglm_create_from_dir: null argument");
+  std::string d(datain_dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  if (p->N < 1 || p->N > 14) return fail(DGLM_ERR_ARG, "dglm_create_from_dir: bad N");
+  const int Np = tet_np(p->N), Nfp = tri_np(p->N);
+  std::vector<int> dim;
+  int rc;
+  if ((rc = read_file(d + "dim.bin", dim, 1))) return rc;
+  const int K = dim[0];
+  if (K <= 0) return fail(DGLM_ERR_IO, "dim.bin: non-positive element count");
+  std::vector<int> EtoB, vmapP, vmapF, Fmask;
+  std::vector<double> WDr, WDs, WDt, LIFT, Fscale, mt[9], nx, ny, nz, px, py, pz;
+  const char *mnames[9] = {"rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz"};
+  if ((rc = read_file(d + "EtoB.bin", EtoB, (size_t)K * 4)) || (rc = read_file(d + "vmapP.bin", vmapP, (size_t)K * 4 * Nfp)) ||
+      (rc = read_file(d + "vmapF.bin", vmapF, (size_t)K * 4)) || (rc = read_file(d + "Fmask.bin", Fmask, 4 * Nfp)) ||
+      (rc = read_file(d + "WDr.bin", WDr, (size_t)Np * Np)) || (rc = read_file(d + "WDs.bin", WDs, (size_t)Np * Np)) ||
+      (rc = read_file(d + "WDt.bin", WDt, (size_t)Np * Np)) || (rc = read_file(d + "LIFT.bin", LIFT, (size_t)Np * 4 * Nfp)) ||
+      (rc = read_file(d + "Fscale.bin", Fscale, (size_t)K * 4)) || (rc = read_file(d + "nx.bin", nx, (size_t)K * 4)) ||
+      (rc = read_file(d + "ny.bin", ny, (size_t)K * 4)) || (rc = read_file(d + "nz.bin", nz, (size_t)K * 4)) ||
+      (rc = read_file(d + "px.bin", px, (size_t)K * Np)) || (rc = read_file(d + "py.bin", py, (size_t)K * Np)) ||
+      (rc = read_file(d + "pz.bin", pz, (size_t)K * Np)))
+    return rc;
+  for (int q = 0; q < 9; q++)
+    if ((rc = read_file(d + mnames[q] + ".bin", mt[q], (size_t)K))) return rc;
+  dglm_mesh m{};
+  m.K = K;
+  m.matrix_order = matrix_order;
+  m.EtoB = EtoB.data();
+  m.vmapP = vmapP.data();
+  m.vmapF = vmapF.data();
+  m.Fmask = Fmask.data();
+  m.WDr = WDr.data();
+  m.WDs = WDs.data();
+  m.WDt = WDt.data();
+  m.LIFT = LIFT.data();
+  m.Fscale = Fscale.data();
+  m.rx = mt[0].data();
+  m.ry = mt[1].data();
+  m.rz = mt[2].data();
+  m.sx = mt[3].data();
+  m.sy = mt[4].data();
+  m.sz = mt[5].data();
+  m.tx = mt[6].data();
+  m.ty = mt[7].data();
+  m.tz = mt[8].data();
+  m.nx = nx.data();
+  m.ny = ny.data();
+  m.nz = nz.data();
+  m.px = px.data();
+  m.py = py.data();
+  m.pz = pz.data();
+  return dglm_create(p, &m, out);
+}
+
+extern "C" int dglm_destroy(dglm_ctx *c) {
+  free_ctx(c);
+  return DGLM_OK;
+}
+
+extern "C" int dglm_sizes(const dglm_ctx *c, int *K, int *Np, int *Nfp, int *nvar) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_sizes: null context");
+  if (K) *K = c->K;
+  if (Np) *Np = c->Np;
+  if (Nfp) *Nfp = c->Nfp;
+  if (nvar) *nvar = c->nvar;
+  return DGLM_OK;
+}
+
+static int set_state_impl(dglm_ctx *c, const double *u, hipMemcpyKind kind) {
+  if (!c || !u) return fail(DGLM_ERR_ARG, "dglm_set_state: null argument");
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->nvar * c->K * c->Np;
+  c->cur = 0;
+  HIP_TRY(hipMemcpyAsync(c->u[0], u, n * sizeof(double), kind, c->stream));
+  HIP_TRY(hipMemsetAsync(c->k, 0, n * sizeof(double), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}
+
+extern "C" int dglm_set_state(dglm_ctx *c, const double *u) { return set_state_impl(c, u, hipMemcpyHostToDevice); }
+extern "C" int dglm_set_state_device(dglm_ctx *c, const double *d_u) {
+  return set_state_impl(c, d_u, hipMemcpyDeviceToDevice);
+}
+
+static int get_state_impl(dglm_ctx *c, double *u, hipMemcpyKind kind) {
+  if (!c || !u) return fail(DGLM_ERR_ARG, "dglm_get_state: null argument");
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->nvar * c->K * c->Np;
+  HIP_TRY(hipMemcpyAsync(u, c->u[c->cur], n * sizeof(double), kind, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}
+
+extern "C" int dglm_get_state(dglm_ctx *c, double *u) { return get_state_impl(c, u, hipMemcpyDeviceToHost); }
+extern "C" int dglm_get_state_device(dglm_ctx *c, double *d_u) {
+  return get_state_impl(c, d_u, hipMemcpyDeviceToDevice);
+}
+
+extern "C" int dglm_set_velocity(dglm_ctx *c, const double *vx, const double *vy, const double *vz) {
+  if (!c || !vx || !vy || !vz) return fail(DGLM_ERR_ARG, "dglm_set_velocity: null argument");
+  if (!c->vel) return fail(DGLM_ERR_ARG, "dglm_set_velocity: only for advection");
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->K * c->Np;
+  HIP_TRY(hipMemcpy(c->vel, vx, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->vel + n, vy, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->vel + 2 * n, vz, n * sizeof(double), hipMemcpyHostToDevice));
+  return DGLM_OK;
+}
+
+extern "C" int dglm_set_time(dglm_ctx *c, double t, int tstep) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_set_time: null context");
+  HIP_TRY(hipSetDevice(c->dev));
+  DevScalars s{};
+  s.t = t;
+  s.t_next = t;
+  s.tstep = tstep;
+  s.active = 1;
+  HIP_TRY(hipMemcpyAsync(c->scal, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}
+
+extern "C" int dglm_get_time(dglm_ctx *c, double *t, double *dt, int *tstep, int *done) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_get_time: null context");
+  HIP_TRY(hipSetDevice(c->dev));
+  DevScalars s;
+  HIP_TRY(hipMemcpyAsync(&s, c->scal, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (t) *t = s.t_next;
+  if (dt) *dt = s.dt;
+  if (tstep) *tstep = s.tstep;
+  if (done) *done = !(s.t_next < c->p.tmax);
+  if (s.bad) return fail(DGLM_ERR_STATE, "non-finite or non-positive dt encountered (state diverged)");
+  return DGLM_OK;
+}
+
+extern "C" int dglm_rhs(dglm_ctx *c, const double *u, double *rhs, double *maxv, double *tau) {
+  if (!c || !u || !rhs) return fail(DGLM_ERR_ARG, "dglm_rhs: null argument");
+  ws_fn ws;
+  st_fn st0, st1;
+  if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_rhs: order not built");
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->nvar * c->K * c->Np;
+  if (!c->rhs) {
+    int rc = dev_alloc(&c->rhs, n, "rhs");
+    if (rc) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(c->u[c->cur], u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ws(c);
+  StageArgs a = stage_args(c, -1);
+  st1(c, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(rhs, c->rhs, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> lam;
+  if (maxv || tau) {
+    lam.resize((size_t)c->K * 4);
+    HIP_TRY(hipMemcpyAsync(lam.data(), c->maxv, lam.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (maxv) memcpy(maxv, lam.data(), lam.size() * sizeof(double));
+  if (tau)
+    for (int o = 0; o < c->K * 4; o++) {
+      int cc = c->h_conn[o];
+      tau[o] = 0.25 * (lam[o] + lam[conn_nbr(cc) * 4 + conn_face(cc)]) * c->p.tau_scale;
+    }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
+  if (!c || nsteps < 0) return fail(DGLM_ERR_ARG, "dglm_step: bad arguments");
+  ws_fn ws;
+  st_fn st0, st1;
+  if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_step: order not built");
+  HIP_TRY(hipSetDevice(c->dev));
+  if (dt_hist && nsteps > c->dt_hist_cap) {
+    if (c->dt_hist) HIP_TRY(hipFree(c->dt_hist));
+    c->dt_hist = nullptr;
+    int rc = dev_alloc(&c->dt_hist, (size_t)nsteps, "dt_hist");
+    if (rc) return rc;
+    c->dt_hist_cap = nsteps;
+  }
+  const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
+  for (int st = 0; st < nsteps; st++) {
+    int tk = timer_start(c, 0);
+    ws(c);
+    timer_stop(c, tk);
+    tk = timer_start(c, 1);
+    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->nblk_ws, c->scal,
+                       cfl_fac, c->p.tmax, dt_hist ? c->dt_hist : nullptr, st);
+    timer_stop(c, tk);
+    for (int s = 0; s < 5; s++) {
+      StageArgs a = stage_args(c, s);
+      tk = timer_start(c, 2);
+      st0(c, a);
+      timer_stop(c, tk);
+      c->cur ^= 1;
+    }
+    if (c->ev_class.size() > 8192) {
+      int rc = timer_flush(c);
+      if (rc) return rc;
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  if (dt_hist) {
+    HIP_TRY(hipMemcpyAsync(dt_hist, c->dt_hist, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_integrals(dglm_ctx *c, const double *J, const double *M, double *out) {
+  if (!c || !J || !M || !out) return fail(DGLM_ERR_ARG, "dglm_integrals: null argument");
+  const int Np = c->Np, K = c->K;
+  std::vector<double> u((size_t)c->nvar * K * Np), w(Np, 0.0);
+  int rc = dglm_get_state(c, u.data());
+  if (rc) return rc;
+  for (int i = 0; i < Np; i++)
+    for (int j = 0; j < Np; j++) w[i] += M[(size_t)j * Np + i];
+  for (int v = 0; v < c->nvar; v++) {
+    double s = 0.0;
+    for (int e = 0; e < K; e++) {
+      double se = 0.0;
+      const double *ue = &u[((size_t)v * K + e) * Np];
+      for (int i = 0; i < Np; i++) se += w[i] * ue[i];
+      s += J[e] * se;
+    }
+    out[v] = s;
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_set_timing(dglm_ctx *c, int on) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_set_timing: null context");
+  int rc = timer_flush(c);
+  if (rc) return rc;
+  c->timing = on != 0;
+  for (int i = 0; i < DGLM_NTIMERS; i++) {
+    c->t_ms[i] = 0;
+    c->t_cnt[i] = 0;
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_get_timing(dglm_ctx *c, double *ms, long *count, int n) {
+  if (!c || !ms || !count) return fail(DGLM_ERR_ARG, "dglm_get_timing: null argument");
+  int rc = timer_flush(c);
+  if (rc) return rc;
+  for (int i = 0; i < n && i < DGLM_NTIMERS; i++) {
+    ms[i] = c->t_ms[i];
+    count[i] = c->t_cnt[i];
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_sync(dglm_ctx *c) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_sync: null context");
+  HIP_TRY(hipSetDevice(c->dev));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}

@@ -1,0 +1,147 @@
+"""Python handle on one solver context of libexadglm_hip.so."""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi as A
+
+_PHYS = {"advection": A.DGLM_ADVECTION, "euler": A.DGLM_EULER}
+_PREC = {"fp64": A.DGLM_FP64, "fp32": A.DGLM_FP32, "bf16x3": A.DGLM_BF16X3}
+
+
+def make_mesh_struct(arrays, K, matrix_order="c"):
+    """dglm_mesh view of a datain-style dict (the arrays must outlive the call)."""
+    m = A.Mesh()
+    m.K = K
+    m.matrix_order = 0 if matrix_order == "c" else 1
+    for n in ("EtoB", "vmapP", "vmapF", "Fmask"):
+        setattr(m, n, A.iptr(arrays[n]) if n in arrays else None)
+    for n in ("WDr", "WDs", "WDt", "LIFT", "Fscale", "rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "nx",
+              "ny", "nz", "px", "py", "pz"):
+        setattr(m, n, A.dptr(arrays[n]) if n in arrays else None)
+    return m
+
+
+class Solver:
+    """Owns a device context: state, LSRK register and the step loop.
+
+    physics: 'advection' or 'euler'; precision: 'fp64' (default).
+    tmax: the run ends exactly at tmax (use math.inf for fixed step counts).
+    """
+
+    def __init__(self, problem=None, physics="euler", N=None, cfl=0.8, gamma=1.4, tau_scale=1.2, tmax=math.inf,
+                 precision="fp64", device=0, arrays=None, K=None, datain_dir=None, matrix_order="c"):
+        L = A.lib()
+        p = A.Params()
+        p.N = N if N is not None else problem.N
+        p.physics = _PHYS[physics]
+        p.precision = _PREC[precision]
+        p.flux = A.DGLM_FLUX_LM
+        p.cfl, p.gamma, p.tau_scale, p.tmax = cfl, gamma, tau_scale, tmax
+        p.device = device
+        self.params = p
+        self.physics = physics
+        self._ctx = C.c_void_p()
+        if datain_dir is not None:
+            A.check(L.dglm_create_from_dir(C.byref(p), datain_dir.encode(), 0 if matrix_order == "c" else 1,
+                                           C.byref(self._ctx)))
+        else:
+            arrays = arrays if arrays is not None else problem.arrays
+            K = K if K is not None else problem.K
+            m = make_mesh_struct(arrays, K, matrix_order)
+            A.check(L.dglm_create(C.byref(p), C.byref(m), C.byref(self._ctx)))
+        K_, Np_, Nfp_, nv_ = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        A.check(L.dglm_sizes(self._ctx, C.byref(K_), C.byref(Np_), C.byref(Nfp_), C.byref(nv_)))
+        self.K, self.Np, self.Nfp, self.nvar = K_.value, Np_.value, Nfp_.value, nv_.value
+
+    @property
+    def shape(self):
+        return (self.nvar, self.K * self.Np)
+
+    def close(self):
+        if self._ctx:
+            A.lib().dglm_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _state_arg(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != self.nvar * self.K * self.Np:
+            raise ValueError(f"state must have {self.nvar}x{self.K * self.Np} entries, got {u.shape}")
+        return u
+
+    def set_state(self, u):
+        u = self._state_arg(u)
+        A.check(A.lib().dglm_set_state(self._ctx, A.dptr(u)))
+
+    def get_state(self):
+        u = np.empty(self.shape)
+        A.check(A.lib().dglm_get_state(self._ctx, A.dptr(u)))
+        return u
+
+    def set_state_device(self, ptr):
+        A.check(A.lib().dglm_set_state_device(self._ctx, C.cast(C.c_void_p(ptr), A.PD)))
+
+    def get_state_device(self, ptr):
+        A.check(A.lib().dglm_get_state_device(self._ctx, C.cast(C.c_void_p(ptr), A.PD)))
+
+    def set_velocity(self, vx, vy, vz):
+        n = self.K * self.Np
+        v = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (vx, vy, vz)]
+        A.check(A.lib().dglm_set_velocity(self._ctx, A.dptr(v[0]), A.dptr(v[1]), A.dptr(v[2])))
+
+    def set_time(self, t=0.0, tstep=0):
+        A.check(A.lib().dglm_set_time(self._ctx, C.c_double(t), tstep))
+
+    def time(self):
+        t, dt, ts, done = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        A.check(A.lib().dglm_get_time(self._ctx, C.byref(t), C.byref(dt), C.byref(ts), C.byref(done)))
+        return t.value, dt.value, ts.value, bool(done.value)
+
+    def rhs(self, u):
+        u = self._state_arg(u)
+        out = np.empty(self.shape)
+        maxv = np.empty((self.K, 4))
+        tau = np.empty((self.K, 4))
+        A.check(A.lib().dglm_rhs(self._ctx, A.dptr(u), A.dptr(out), A.dptr(maxv), A.dptr(tau)))
+        return out, maxv, tau
+
+    def step(self, nsteps=1, return_dt=False):
+        if return_dt:
+            dts = np.empty(nsteps)
+            A.check(A.lib().dglm_step(self._ctx, nsteps, A.dptr(dts)))
+            return dts
+        A.check(A.lib().dglm_step(self._ctx, nsteps, None))
+        return None
+
+    def set_timing(self, on=True):
+        A.check(A.lib().dglm_set_timing(self._ctx, 1 if on else 0))
+
+    def timing(self):
+        """{'wavespeed'|'dt'|'stage': (total_ms, launches)} since set_timing(True)."""
+        ms = (C.c_double * 3)()
+        cnt = (C.c_long * 3)()
+        A.check(A.lib().dglm_get_timing(self._ctx, ms, cnt, 3))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(("wavespeed", "dt", "stage"))}
+
+    def sync(self):
+        A.check(A.lib().dglm_sync(self._ctx))
+
+    def integrals(self, J, M):
+        out = np.empty(self.nvar)
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        A.check(A.lib().dglm_integrals(self._ctx, A.dptr(J), A.dptr(M), A.dptr(out)))
+        return out

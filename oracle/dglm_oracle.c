@@ -1,0 +1,294 @@
+/*
+ * dglm_oracle.c -- CPU restatement of the reference explicit DGLM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP product
+ * path: it is built into oracle/liboracle.so and loaded only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  The product
+ * library (libexadglm_hip.so) never links or calls it.
+ *
+ * It restates, with runtime N / NVAR and OpenMP, the single-partition CPU
+ * solver of the reference:
+ *   calc_maxv    dim3/src/cpp/advec_dglm.cpp:6-35,  euler_dglm.cpp:6-50
+ *   calc_dt      advec_dglm.cpp:38-53,              euler_dglm.cpp:53-69
+ *   update_tau   advec_dglm.cpp:56-66,              euler_dglm.cpp:72-84
+ *   update_ub    advec_dglm.cpp:69-85,              euler_dglm.cpp:87-107
+ *   calc_fluxLM  advec_dglm.cpp:88-133,             euler_dglm.cpp:110-183
+ *   calc_fluxes  advec_dglm.cpp:136-169,            euler_dglm.cpp:186-251
+ *   update_rhs   advec_dglm.cpp:172-195,            euler_dglm.cpp:254-302
+ *   lserk_stage  advec_time.cpp:15-26,              euler_time.cpp:15-38
+ *   runRK        advec_time.cpp:29-97,              euler_time.cpp:41-107
+ * with the same floating-point expression order, so that it agrees with the
+ * compiled reference to rounding (pinned in tests/test_oracle_ref.py).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "dglm_oracle.h"
+
+static const double RKA[5] = {0.0, -567301805773.0 / 1357537059087.0, -2404267990393.0 / 2016746695238.0,
+                              -3550918686646.0 / 2091501179385.0, -1275806237668.0 / 842570457699.0};
+static const double RKB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+
+#define NFACE 4
+
+static void calc_maxv(const oracle_problem *P, const double *u, double *maxv, double *maxv_fs) {
+  const int K = P->K, Np = P->Np, Nfp = P->Nfp;
+  const size_t KNp = (size_t)K * Np;
+  const double G = P->gamma;
+#pragma omp parallel for
+  for (int fidx = 0; fidx < K * NFACE; fidx++) {
+    int ei = fidx / NFACE, fi = fidx % NFACE;
+    double max_v2 = 0, max_c2 = 0;
+    for (int j = 0; j < Nfp; j++) {
+      size_t idxM = (size_t)ei * Np + P->Fmask[fi * Nfp + j];
+      if (P->nvar == 5) {
+        double u1 = u[idxM], u2 = u[KNp + idxM], u3 = u[2 * KNp + idxM], u4 = u[3 * KNp + idxM],
+               u5 = u[4 * KNp + idxM];
+        double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
+        double v2 = vx * vx + vy * vy + vz * vz;
+        if (v2 > max_v2) max_v2 = v2;
+        double pres = (G - 1) * (u5 - 0.5 * u1 * v2);
+        double sound_c2 = fabs(G * pres / u1);
+        if (sound_c2 > max_c2) max_c2 = sound_c2;
+      } else {
+        double vx = P->vx[idxM], vy = P->vy[idxM], vz = P->vz[idxM];
+        double v2 = vx * vx + vy * vy + vz * vz;
+        if (v2 > max_v2) max_v2 = v2;
+      }
+    }
+    double mv = P->nvar == 5 ? sqrt(max_v2) + sqrt(max_c2) : sqrt(max_v2);
+    maxv[fidx] = mv;
+    maxv_fs[fidx] = mv * P->Fscale[fidx];
+  }
+}
+
+static double calc_dt(const oracle_problem *P, const double *maxv_fs) {
+  double max_v = 0;
+#pragma omp parallel for reduction(max : max_v)
+  for (int fidx = 0; fidx < P->K * NFACE; fidx++) {
+    double v = maxv_fs[fidx];
+    if (v > max_v) max_v = v;
+  }
+  return (P->cfl / (2 * P->N + 1)) * (1 / max_v);
+}
+
+static void update_tau(const oracle_problem *P, const double *maxv, double *tau) {
+#pragma omp parallel for
+  for (int fidx = 0; fidx < P->K * NFACE; fidx++)
+    tau[fidx] = 0.25 * (maxv[fidx] + maxv[P->vmapF[fidx]]) * P->tau_scale;
+}
+
+static void update_ub(const oracle_problem *P, const double *u, double *ub) {
+  const int K = P->K, Np = P->Np, Nfp = P->Nfp, NFF = NFACE * Nfp;
+  const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
+#pragma omp parallel for
+  for (long idx = 0; idx < (long)K * NFF; idx++) {
+    int ei = (int)(idx / NFF), fi = (int)((idx / Nfp) % NFACE);
+    size_t idxM = (size_t)ei * Np + P->Fmask[fi * Nfp + idx % Nfp];
+    size_t idxP = (size_t)P->vmapP[idx];
+    for (int v = 0; v < P->nvar; v++) ub[v * KNF + idx] = 0.5 * (u[v * KNp + idxM] + u[v * KNp + idxP]);
+  }
+}
+
+static void calc_fluxLM(const oracle_problem *P, const double *tau, const double *ub, const double *u,
+                        double *fluxLM) {
+  const int K = P->K, Np = P->Np, Nfp = P->Nfp, NFF = NFACE * Nfp;
+  const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
+  const double G = P->gamma;
+#pragma omp parallel for
+  for (int fidx = 0; fidx < K * NFACE; fidx++) {
+    double nx = P->nx[fidx], ny = P->ny[fidx], nz = P->nz[fidx], fs = P->Fscale[fidx];
+    int ei = fidx / NFACE, fi = fidx % NFACE;
+    for (int j = 0; j < Nfp; j++) {
+      size_t idx = (size_t)fidx * Nfp + j;
+      size_t idxM = (size_t)ei * Np + P->Fmask[fi * Nfp + j];
+      if (P->nvar == 5) {
+        double ub1 = ub[idx], ub2 = ub[KNF + idx], ub3 = ub[2 * KNF + idx], ub4 = ub[3 * KNF + idx],
+               ub5 = ub[4 * KNF + idx];
+        double vb2 = ub2 * ub2 + ub3 * ub3 + ub4 * ub4;
+        double F[5], Gf[5], W[5];
+        F[0] = ub2;
+        F[1] = (ub2 * ub2) / ub1 + (G - 1) * (ub5 - 0.5 * vb2 / ub1);
+        F[2] = (ub2 * ub3) / ub1;
+        F[3] = (ub2 * ub4) / ub1;
+        F[4] = (ub5 + (G - 1) * (ub5 - 0.5 * vb2 / ub1)) * (ub2 / ub1);
+        Gf[0] = ub3;
+        Gf[1] = ub2 * ub3 / ub1;
+        Gf[2] = (ub3 * ub3) / ub1 + (G - 1) * (ub5 - 0.5 * vb2 / ub1);
+        Gf[3] = ub3 * ub4 / ub1;
+        Gf[4] = (ub5 + (G - 1) * (ub5 - 0.5 * vb2 / ub1)) * (ub3 / ub1);
+        W[0] = ub4;
+        W[1] = ub2 * ub4 / ub1;
+        W[2] = ub3 * ub4 / ub1;
+        W[3] = (ub4 * ub4) / ub1 + (G - 1) * (ub5 - 0.5 * vb2 / ub1);
+        W[4] = (ub5 + (G - 1) * (ub5 - 0.5 * vb2 / ub1)) * (ub4 / ub1);
+        double ubv[5] = {ub1, ub2, ub3, ub4, ub5};
+        for (int v = 0; v < 5; v++) {
+          double nf = nx * F[v] + ny * Gf[v] + nz * W[v];
+          fluxLM[v * KNF + idx] = (nf - tau[fidx] * (ubv[v] - u[v * KNp + idxM])) * fs;
+        }
+      } else {
+        double b = ub[idx];
+        double fF = P->vx[idxM] * b, fG = P->vy[idxM] * b, fW = P->vz[idxM] * b;
+        double nf = nx * fF + ny * fG + nz * fW;
+        fluxLM[idx] = (nf - tau[fidx] * (b - u[idxM])) * fs;
+      }
+    }
+  }
+}
+
+static void calc_fluxes(const oracle_problem *P, const double *u, double *fR, double *fS, double *fT) {
+  const int K = P->K, Np = P->Np;
+  const size_t KNp = (size_t)K * Np;
+  const double G = P->gamma;
+#pragma omp parallel for
+  for (int ei = 0; ei < K; ei++) {
+    double rx = P->rx[ei], ry = P->ry[ei], rz = P->rz[ei];
+    double sx = P->sx[ei], sy = P->sy[ei], sz = P->sz[ei];
+    double tx = P->tx[ei], ty = P->ty[ei], tz = P->tz[ei];
+    for (int i = 0; i < Np; i++) {
+      size_t idx = (size_t)ei * Np + i;
+      if (P->nvar == 5) {
+        double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx], u5 = u[4 * KNp + idx];
+        double v2 = u2 * u2 + u3 * u3 + u4 * u4;
+        double F[5], Gf[5], W[5];
+        F[0] = u2;
+        F[1] = (u2 * u2) / u1 + (G - 1) * (u5 - 0.5 * v2 / u1);
+        F[2] = (u2 * u3) / u1;
+        F[3] = (u2 * u4) / u1;
+        F[4] = (u5 + (G - 1) * (u5 - 0.5 * v2 / u1)) * (u2 / u1);
+        Gf[0] = u3;
+        Gf[1] = u2 * u3 / u1;
+        Gf[2] = (u3 * u3) / u1 + (G - 1) * (u5 - 0.5 * v2 / u1);
+        Gf[3] = u3 * u4 / u1;
+        Gf[4] = (u5 + (G - 1) * (u5 - 0.5 * v2 / u1)) * (u3 / u1);
+        W[0] = u4;
+        W[1] = u2 * u4 / u1;
+        W[2] = u3 * u4 / u1;
+        W[3] = (u4 * u4) / u1 + (G - 1) * (u5 - 0.5 * v2 / u1);
+        W[4] = (u5 + (G - 1) * (u5 - 0.5 * v2 / u1)) * (u4 / u1);
+        for (int v = 0; v < 5; v++) {
+          fR[v * KNp + idx] = rx * F[v] + ry * Gf[v] + rz * W[v];
+          fS[v * KNp + idx] = sx * F[v] + sy * Gf[v] + sz * W[v];
+          fT[v * KNp + idx] = tx * F[v] + ty * Gf[v] + tz * W[v];
+        }
+      } else {
+        double fF = P->vx[idx] * u[idx], fG = P->vy[idx] * u[idx], fW = P->vz[idx] * u[idx];
+        fR[idx] = rx * fF + ry * fG + rz * fW;
+        fS[idx] = sx * fF + sy * fG + sz * fW;
+        fT[idx] = tx * fF + ty * fG + tz * fW;
+      }
+    }
+  }
+}
+
+static void update_rhs(const oracle_problem *P, const double *fR, const double *fS, const double *fT,
+                       const double *fLM, double *rhs) {
+  const int K = P->K, Np = P->Np, NFF = NFACE * P->Nfp;
+  const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
+#pragma omp parallel for
+  for (int ei = 0; ei < K; ei++) {
+    for (int i = 0; i < Np; i++) {
+      for (int v = 0; v < P->nvar; v++) {
+        const double *r = fR + v * KNp + (size_t)ei * Np, *s = fS + v * KNp + (size_t)ei * Np,
+                     *t = fT + v * KNp + (size_t)ei * Np, *l = fLM + v * KNF + (size_t)ei * NFF;
+        double acc = 0;
+        for (int j = 0; j < Np; j++)
+          acc += P->WDr[i * Np + j] * r[j] + P->WDs[i * Np + j] * s[j] + P->WDt[i * Np + j] * t[j];
+        for (int j = 0; j < NFF; j++) acc -= P->LIFT[i * NFF + j] * l[j];
+        rhs[v * KNp + (size_t)ei * Np + i] = acc;
+      }
+    }
+  }
+}
+
+typedef struct {
+  double *maxv, *maxv_fs, *tau, *ub, *fR, *fS, *fT, *fLM;
+} work_t;
+
+static int work_alloc(const oracle_problem *P, work_t *w) {
+  const size_t KNp = (size_t)P->K * P->Np, KNF = (size_t)P->K * NFACE * P->Nfp, KF = (size_t)P->K * NFACE;
+  memset(w, 0, sizeof(*w));
+  w->maxv = malloc(KF * sizeof(double));
+  w->maxv_fs = malloc(KF * sizeof(double));
+  w->tau = malloc(KF * sizeof(double));
+  w->ub = malloc(P->nvar * KNF * sizeof(double));
+  w->fLM = malloc(P->nvar * KNF * sizeof(double));
+  w->fR = malloc(P->nvar * KNp * sizeof(double));
+  w->fS = malloc(P->nvar * KNp * sizeof(double));
+  w->fT = malloc(P->nvar * KNp * sizeof(double));
+  return (w->maxv && w->maxv_fs && w->tau && w->ub && w->fLM && w->fR && w->fS && w->fT) ? 0 : -1;
+}
+
+static void work_free(work_t *w) {
+  free(w->maxv);
+  free(w->maxv_fs);
+  free(w->tau);
+  free(w->ub);
+  free(w->fLM);
+  free(w->fR);
+  free(w->fS);
+  free(w->fT);
+}
+
+static void rhs_core(const oracle_problem *P, work_t *w, const double *u, double *rhs) {
+  update_ub(P, u, w->ub);
+  calc_fluxLM(P, w->tau, w->ub, u, w->fLM);
+  calc_fluxes(P, u, w->fR, w->fS, w->fT);
+  update_rhs(P, w->fR, w->fS, w->fT, w->fLM, rhs);
+}
+
+int oracle_rhs(const oracle_problem *P, const double *u, double *rhs, double *maxv, double *tau) {
+  work_t w;
+  if (work_alloc(P, &w)) {
+    work_free(&w);
+    return -1;
+  }
+  calc_maxv(P, u, w.maxv, w.maxv_fs);
+  update_tau(P, w.maxv, w.tau);
+  rhs_core(P, &w, u, rhs);
+  const size_t KF = (size_t)P->K * NFACE;
+  if (maxv) memcpy(maxv, w.maxv, KF * sizeof(double));
+  if (tau) memcpy(tau, w.tau, KF * sizeof(double));
+  work_free(&w);
+  return 0;
+}
+
+int oracle_run(const oracle_problem *P, double *u, double *k, int nsteps, double *t, int *tstep, double *dt_hist) {
+  work_t w;
+  const size_t n = (size_t)P->nvar * P->K * P->Np;
+  double *utmp = malloc(n * sizeof(double));
+  if (work_alloc(P, &w) || !utmp) {
+    work_free(&w);
+    free(utmp);
+    return -1;
+  }
+  for (int st = 0; st < nsteps; st++) {
+    if (!(*t < P->tmax)) {
+      if (dt_hist) dt_hist[st] = 0.0;
+      continue;
+    }
+    calc_maxv(P, u, w.maxv, w.maxv_fs);
+    double dt = calc_dt(P, w.maxv_fs);
+    if (*t + dt > P->tmax) dt = P->tmax - *t;
+    update_tau(P, w.maxv, w.tau);
+    for (int s = 0; s < 5; s++) {
+      rhs_core(P, &w, u, utmp);
+      const double a = RKA[s], b = RKB[s];
+#pragma omp parallel for
+      for (long i = 0; i < (long)n; i++) {
+        k[i] = a * k[i] + dt * utmp[i];
+        u[i] += b * k[i];
+      }
+    }
+    if (dt_hist) dt_hist[st] = dt;
+    *t += dt;
+    *tstep += 1;
+  }
+  work_free(&w);
+  free(utmp);
+  return 0;
+}

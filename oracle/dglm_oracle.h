@@ -1,0 +1,36 @@
+/*
+ * dglm_oracle.h -- CPU restatement of the reference DGLM RHS + LSRK path.
+ * TEST INFRASTRUCTURE ONLY (the checker; never part of the product library).
+ */
+#ifndef DGLM_ORACLE_H
+#define DGLM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The datain/ array set of the reference (dglm_3d_launcher.py:74-149) with
+ * operators row-major, plus the advection velocity per node. */
+typedef struct oracle_problem {
+  int N, Np, Nfp, K, nvar;
+  double cfl, gamma, tau_scale, tmax;
+  const int *vmapP, *vmapF, *Fmask;
+  const double *WDr, *WDs, *WDt, *LIFT;
+  const double *Fscale;
+  const double *rx, *ry, *rz, *sx, *sy, *sz, *tx, *ty, *tz;
+  const double *nx, *ny, *nz;
+  const double *vx, *vy, *vz;
+} oracle_problem;
+
+/* One RHS at state u ([nvar][K*Np]); maxv, tau [K*4] optional. */
+int oracle_rhs(const oracle_problem *P, const double *u, double *rhs, double *maxv, double *tau);
+
+/* nsteps of runRK on (u, k) in place; t and tstep advance; steps once t has
+ * reached tmax are skipped (dt_hist entry 0). */
+int oracle_run(const oracle_problem *P, double *u, double *k, int nsteps, double *t, int *tstep, double *dt_hist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,199 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bounds: the FP64 kernels sum the contraction in a different order from the
+oracle (and use FMA), so single RHS evaluations agree to ~1e-15 relative and
+runs of 50-200 steps to ~1e-13; the required bound is 1e-12 relative L2 per
+conserved variable."""
+import math
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from helpers import rel_err, rel_err_per_var, smooth_state
+from exadglm_amd.problem import (cube_problem, ic_advection_sine, ic_euler_density_wave, isentropic_vortex, l2_error,
+                                 uniform_problem, vortex_box)
+from exadglm_amd.solver import Solver
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+PHYS = ["advection", "euler"]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("physics", PHYS)
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_rhs_parity(gpu, physics, N):
+    _say(f"rhs parity {physics} N={N}")
+    pb = cube_problem(N, 3)
+    u = smooth_state(pb, physics, seed=N + 11)
+    with Solver(pb, physics) as s:
+        r, mv, tau = s.rhs(u)
+    ro, mvo, tauo = O.Oracle(pb, physics).rhs(u)
+    assert rel_err(mv, mvo) < 1e-15
+    assert rel_err(tau, tauo) < 1e-15
+    for e in rel_err_per_var(r, ro):
+        assert e < 1e-13
+
+
+def test_c1_advection_100_steps(gpu):
+    """Linear advection N=3 on 10^3 hexes x 6 = 6000 tets, v=(1,1,1),
+    sin sin sin, CFL 0.5, 100 steps."""
+    _say("C1 advection K=6000 N=3 100 steps")
+    pb = cube_problem(3, 10)
+    assert pb.K == 6000
+    u0 = ic_advection_sine(pb.arrays["px"], pb.arrays["py"], pb.arrays["pz"])
+    with Solver(pb, "advection", cfl=0.5) as s:
+        s.set_state(u0)
+        dts = s.step(100, return_dt=True)
+        u = s.get_state()
+        t, _, ts, _ = s.time()
+    uo, dto, to = O.Oracle(pb, "advection", cfl=0.5).run(u0, 100)
+    assert ts == 100
+    assert rel_err(dts, dto) < 1e-14
+    assert abs(t - to) < 1e-13 * to
+    assert rel_err(u, uo) < 1e-12
+
+
+@pytest.mark.parametrize("N", [3, 4, 5, 6])
+def test_euler_vortex_steps(gpu, N):
+    _say(f"Euler vortex N={N} 6^3 hexes 40 steps")
+    pb = uniform_problem(N, 7, 7, 7, box=vortex_box(6))
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    with Solver(pb, "euler", cfl=0.4) as s:
+        s.set_state(u0)
+        dts = s.step(40, return_dt=True)
+        u = s.get_state()
+    uo, dto, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 40)
+    assert rel_err(dts, dto) < 1e-13
+    for e in rel_err_per_var(u, uo):
+        assert e < 1e-12
+
+
+def test_advection_velocity_field(gpu):
+    """Advection with v = (1, 0.5, 0.25), N=5."""
+    _say("advection v=(1,.5,.25) N=5")
+    pb = cube_problem(5, 4)
+    u0 = smooth_state(pb, "advection", seed=4)
+    v = (1.0, 0.5, 0.25)
+    with Solver(pb, "advection", cfl=0.5) as s:
+        s.set_velocity(*v)
+        s.set_state(u0)
+        s.step(30)
+        u = s.get_state()
+    uo, _, _ = O.Oracle(pb, "advection", cfl=0.5, velocity=v).run(u0, 30)
+    assert rel_err(u, uo) < 1e-12
+
+
+def test_c2_full_size_properties(gpu):
+    """Euler N=4 on 32^3 hexes x 6 = 196,608 tets (the benchmark workload):
+    3 steps against the oracle, then conservation of mass, momentum and
+    energy over 20 more steps, and bitwise determinism of a rerun."""
+    _say("C2 full size K=196608 N=4")
+    pb = uniform_problem(4, 33, 33, 33, box=vortex_box(32))
+    assert pb.K == 196608
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    with Solver(pb, "euler", cfl=0.4) as s:
+        s.set_state(u0)
+        s.step(3)
+        u3 = s.get_state()
+        I0 = s.integrals(pb.mesh.J, pb.op_ref.M)
+        s.step(20)
+        I1 = s.integrals(pb.mesh.J, pb.op_ref.M)
+        u23 = s.get_state()
+        s.set_state(u0)
+        s.step(23)
+        u23b = s.get_state()
+    _say("  oracle 3 steps")
+    uo, _, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 3)
+    for e in rel_err_per_var(u3, uo):
+        assert e < 1e-12
+    for v in (0, 1, 2, 4):
+        assert abs(I1[v] - I0[v]) <= 1e-12 * abs(I0[v])
+    assert np.array_equal(u23, u23b)
+
+
+def _adv_exact(pb, t):
+    m = pb.mesh
+    x, y, z = m.high_PX.ravel(), m.high_PY.ravel(), m.high_PZ.ravel()
+    xv, yv, zv = ((x - t) + 1) % 1 - 1, ((y - t) + 1) % 1 - 1, ((z - t) + 1) % 1 - 1
+    return np.sin(2 * np.pi * xv) * np.sin(2 * np.pi * yv) * np.sin(2 * np.pi * zv)
+
+
+def test_published_l2_advection(gpu):
+    """ex1.advection.11.cpu.single.ipynb:214: 809 steps, L2 = 2.284502600150407e-05."""
+    _say("published advection L2")
+    pb = cube_problem(5, 6)
+    u0 = ic_advection_sine(pb.arrays["px"], pb.arrays["py"], pb.arrays["pz"])
+    with Solver(pb, "advection", cfl=0.8, tmax=2.0) as s:
+        s.set_state(u0)
+        dts = s.step(850, return_dt=True)
+        t, _, ts, done = s.time()
+        u = s.get_state()
+    assert ts == 809 and done and t == 2.0 and int((dts > 0).sum()) == 809
+    l2 = l2_error(pb, _adv_exact(pb, 2.0), u[0])
+    assert abs(l2 - 2.284502600150407e-05) / 2.284502600150407e-05 < 1e-9
+
+
+def test_published_l2_euler(gpu):
+    """ex3.euler.11.cpu.single.ipynb:221: 1318 steps, L2(rho) = 2.429869462976010e-05."""
+    _say("published Euler L2")
+    pb = cube_problem(5, 6)
+    u0 = ic_euler_density_wave(pb.arrays["px"], pb.arrays["py"], pb.arrays["pz"])
+    with Solver(pb, "euler", cfl=0.8, tmax=2.0) as s:
+        s.set_state(u0)
+        s.step(1400)
+        t, _, ts, done = s.time()
+        u = s.get_state()
+    assert ts == 1318 and done
+    m = pb.mesh
+    x, y, z = m.high_PX.ravel(), m.high_PY.ravel(), m.high_PZ.ravel()
+    xv, yv, zv = ((x - 2) + 1) % 1 - 1, ((y + 1) + 1) % 1 - 1, ((z - 2) + 1) % 1 - 1
+    l2 = l2_error(pb, 1 + 0.2 * np.sin(2 * np.pi * (xv + yv + zv)), u[0])
+    assert abs(l2 - 2.429869462976010e-05) / 2.429869462976010e-05 < 1e-9
+
+
+def test_create_from_datain_dir(gpu):
+    """Drop-in on a work dir written in the reference's datain format, both
+    matrix orders ('c' for the CPU build, 'f' for the cuBLAS build)."""
+    _say("datain dir drop-in")
+    pb = cube_problem(4, 3)
+    u0 = smooth_state(pb, "euler", seed=5)
+    with Solver(pb, "euler", cfl=0.5) as s:
+        s.set_state(u0)
+        s.step(5)
+        ua = s.get_state()
+    for order in ("c", "f"):
+        with tempfile.TemporaryDirectory() as d:
+            O.write_datain(pb, d, matrix_order=order)
+            with Solver(N=4, physics="euler", cfl=0.5, datain_dir=d, matrix_order=order) as s:
+                s.set_state(u0)
+                s.step(5)
+                ub = s.get_state()
+        assert np.array_equal(ua, ub)
+
+
+def test_timing_and_noop_after_tmax(gpu):
+    _say("timing + tmax no-op")
+    pb = cube_problem(3, 3)
+    u0 = smooth_state(pb, "euler", seed=2)
+    with Solver(pb, "euler", cfl=0.5, tmax=0.01) as s:
+        s.set_state(u0)
+        s.set_timing(True)
+        s.step(200)
+        tm = s.timing()
+        t, _, ts, done = s.time()
+        u = s.get_state()
+        s.step(3)
+        assert np.array_equal(u, s.get_state())
+    assert done and t == 0.01
+    assert tm["stage"][1] == 1000 and tm["stage"][0] > 0
+    uo, _, _ = O.Oracle(pb, "euler", cfl=0.5, tmax=0.01).run(u0, 200)
+    assert rel_err(u, uo) < 1e-12

@@ -1,0 +1,145 @@
+"""Pins of the CPU oracle: against the compiled reference solver (oracle/_ref),
+against the reference's published end-to-end L2 errors, and against
+mathematical invariants (free stream, conservation, exact transport)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from helpers import rel_err, smooth_state
+from exadglm_amd.problem import cube_problem, ic_advection_sine, ic_euler_density_wave, l2_error
+import pyoracle as O
+
+PHYS = ["advection", "euler"]
+
+
+def _need_ref(physics, N):
+    if not O.ref_available(physics, N):
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+
+
+@pytest.mark.parametrize("physics", PHYS)
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_rhs_matches_compiled_reference(physics, N):
+    _need_ref(physics, N)
+    pb = cube_problem(N, 2)
+    u = smooth_state(pb, physics, seed=N)
+    o = O.Oracle(pb, physics)
+    r = O.RefSolver(pb, physics)
+    rhs_o, mv_o, tau_o = o.rhs(u)
+    rhs_r, mv_r, tau_r = r.rhs(u)
+    assert rel_err(mv_o, mv_r) < 1e-15
+    assert rel_err(tau_o, tau_r) < 1e-15
+    assert rel_err(rhs_o, rhs_r) < 1e-13
+
+
+@pytest.mark.parametrize("physics,N,nsteps", [("advection", 3, 20), ("euler", 4, 10)])
+def test_steps_match_compiled_reference(physics, N, nsteps):
+    _need_ref(physics, N)
+    pb = cube_problem(N, 3)
+    u0 = smooth_state(pb, physics, seed=3)
+    uo, dto, to = O.Oracle(pb, physics, cfl=0.5).run(u0, nsteps)
+    ur, dtr, tr = O.RefSolver(pb, physics, cfl=0.5).steps(u0, nsteps)
+    assert rel_err(dto, dtr) < 1e-14
+    for v in range(uo.shape[0]):
+        assert rel_err(uo[v], ur[v]) < 1e-13
+
+
+def _advection_exact(pb, t):
+    m = pb.mesh
+    x, y, z = m.high_PX.ravel(), m.high_PY.ravel(), m.high_PZ.ravel()
+    xv, yv, zv = ((x - t) + 1) % 1 - 1, ((y - t) + 1) % 1 - 1, ((z - t) + 1) % 1 - 1
+    return np.sin(2 * np.pi * xv) * np.sin(2 * np.pi * yv) * np.sin(2 * np.pi * zv)
+
+
+def test_published_advection_l2():
+    """ex1.advection.11.cpu.single.ipynb:214 -- K=1296, N=5, CFL 0.8, tmax 2:
+    809 steps, L2 = 2.284502600150407e-05.  The reference's own CPU and GPU
+    runs agree to ~2e-12 relative; machine-level FMA / ulp differences of the
+    tables give ~1e-11 here, so the bound is 1e-9 relative."""
+    pb = cube_problem(5, 6)
+    u0 = ic_advection_sine(pb.arrays["px"], pb.arrays["py"], pb.arrays["pz"])
+    u, dts, t = O.Oracle(pb, "advection", cfl=0.8, tmax=2.0).run(u0, 900)
+    assert int((dts > 0).sum()) == 809 and t == 2.0
+    l2 = l2_error(pb, _advection_exact(pb, 2.0), u[0])
+    assert abs(l2 - 2.284502600150407e-05) / 2.284502600150407e-05 < 1e-9
+
+
+@pytest.mark.slow
+def test_published_euler_l2():
+    """ex3.euler.11.cpu.single.ipynb:221 -- density wave, K=1296, N=5: 1318
+    steps, L2(rho) = 2.429869462976010e-05 (bound as above)."""
+    pb = cube_problem(5, 6)
+    u0 = ic_euler_density_wave(pb.arrays["px"], pb.arrays["py"], pb.arrays["pz"])
+    u, dts, t = O.Oracle(pb, "euler", cfl=0.8, tmax=2.0).run(u0, 1400)
+    assert int((dts > 0).sum()) == 1318
+    m = pb.mesh
+    x, y, z = m.high_PX.ravel(), m.high_PY.ravel(), m.high_PZ.ravel()
+    xv, yv, zv = ((x - 2) + 1) % 1 - 1, ((y + 1) + 1) % 1 - 1, ((z - 2) + 1) % 1 - 1
+    ref = 1 + 0.2 * np.sin(2 * np.pi * (xv + yv + zv))
+    l2 = l2_error(pb, ref, u[0])
+    assert abs(l2 - 2.429869462976010e-05) / 2.429869462976010e-05 < 1e-9
+
+
+@pytest.mark.parametrize("N", [2, 4, 6])
+def test_free_stream_preserved(N):
+    pb = cube_problem(N, 2)
+    n = pb.K * pb.Np
+    o = O.Oracle(pb, "euler")
+    rho, vx, vy, vz, p = 1.3, 0.4, -0.2, 0.7, 2.0
+    u = np.stack([np.full(n, c) for c in (rho, rho * vx, rho * vy, rho * vz,
+                                          p / 0.4 + 0.5 * rho * (vx * vx + vy * vy + vz * vz))])
+    rhs, _, _ = o.rhs(u)
+    # round-off scale: |operator row| x |metric| x |flux|
+    a = pb.arrays
+    opn = max(np.abs(a[k]).sum(axis=1).max() for k in ("WDr", "WDs", "WDt", "LIFT"))
+    met = max(np.abs(a[k]).max() for k in ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "Fscale"))
+    scale = 3 * opn * met * np.abs(u).max()
+    assert np.abs(rhs).max() < 1e-14 * scale
+    oa = O.Oracle(pb, "advection")
+    rhs, _, _ = oa.rhs(np.full((1, n), 2.5))
+    assert np.abs(rhs).max() < 1e-14 * 3 * opn * met * 2.5
+
+
+@pytest.mark.parametrize("physics", PHYS)
+def test_conservation_of_rhs(physics):
+    """sum_e J_e 1^T M rhs_e = 0 on a periodic mesh (flux telescoping)."""
+    pb = cube_problem(3, 3)
+    u = smooth_state(pb, physics, seed=7)
+    rhs, _, _ = O.Oracle(pb, physics).rhs(u)
+    w = pb.op_ref.M.sum(axis=0)
+    for v in range(rhs.shape[0]):
+        tot = (pb.mesh.J[:, None] * (rhs[v].reshape(pb.K, pb.Np) * w)).sum()
+        scale = (pb.mesh.J[:, None] * np.abs(rhs[v].reshape(pb.K, pb.Np)) * np.abs(w)).sum()
+        assert abs(tot) < 1e-13 * scale
+
+
+def test_planted_connectivity_error_is_caught():
+    """A wrong node pairing across faces (vmapP rolled within each face) in the
+    oracle's inputs breaks conservation: the invariant above is sensitive to
+    that class of mistake."""
+    pb = cube_problem(3, 3)
+    u = smooth_state(pb, "euler", seed=7)
+    bad = dict(pb.arrays)
+    bad["vmapP"] = np.ascontiguousarray(np.roll(pb.arrays["vmapP"], 1, axis=2))
+    pb2 = type(pb)(**{**pb.__dict__, "arrays": bad})
+    rhs, _, _ = O.Oracle(pb2, "euler").rhs(u)
+    w = pb.op_ref.M.sum(axis=0)
+    tot = (pb.mesh.J[:, None] * (rhs[0].reshape(pb.K, pb.Np) * w)).sum()
+    scale = (pb.mesh.J[:, None] * np.abs(rhs[0].reshape(pb.K, pb.Np)) * np.abs(w)).sum()
+    assert abs(tot) > 1e-6 * scale
+
+
+def test_advection_period_return():
+    """v=(1,1,1) on the unit cube returns the field to itself at t=1; the DG
+    error converges at high order between two resolutions."""
+    errs = []
+    for nhex in (3, 6):
+        pb = cube_problem(3, nhex)
+        u0 = ic_advection_sine(pb.arrays["px"], pb.arrays["py"], pb.arrays["pz"])
+        u, _, t = O.Oracle(pb, "advection", cfl=0.5, tmax=1.0).run(u0, 10000)
+        assert t == 1.0
+        errs.append(l2_error(pb, _advection_exact(pb, 1.0), u[0]))
+    assert errs[1] < errs[0] / 8  # at least O(h^3)
