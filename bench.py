@@ -75,7 +75,22 @@ def main():
     ap.add_argument("--nhex", type=int, default=32)
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--stage-kernel", default="auto", choices=["auto", "simple", "mfma"])
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (scales the FETCH_SIZE / "
+                         "WRITE_SIZE counters of a profiled run to bytes)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU halo path is not built",
               file=sys.stderr)
@@ -91,6 +106,8 @@ def main():
     K, Np, Nfp, nvar = pb.K, pb.Np, pb.Nfp, 5
 
     s = Solver(pb, "euler", cfl=0.4, tmax=math.inf)
+    s.set_stage_kernel(args.stage_kernel)
+    kernel_name = s.stage_kernel()
     s.set_state(u0)
     s.step(args.warmup)
     s.sync()
@@ -123,7 +140,7 @@ def main():
         roof = {"bound": "mfma", "achieved": tfl, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": tfl / FP64_PEAK_TFLOPS}
     roof["traffic"] = None
-    roof["kernel"] = "stage_simple_kernel<4,EULER,0>"
+    roof["kernel"] = f"stage_{kernel_name}_kernel<{args.order},EULER,0>"
     roof["avg_launch_us"] = st_avg_s * 1e6
     roof["algorithmic_bytes_per_launch"] = B
     roof["contraction_flops_per_launch"] = F

@@ -24,6 +24,9 @@ DGLM_FP64 = 0
 DGLM_FP32 = 1
 DGLM_BF16X3 = 2
 DGLM_FLUX_LM = 0
+DGLM_STAGE_AUTO = 0
+DGLM_STAGE_SIMPLE = 1
+DGLM_STAGE_MFMA = 2
 
 _ERRNAMES = {
     DGLM_ERR_ARG: "DGLM_ERR_ARG",
@@ -70,6 +73,7 @@ EXPORTS = [
     "dglm_destroy", "dglm_sizes", "dglm_set_state", "dglm_get_state", "dglm_set_state_device",
     "dglm_get_state_device", "dglm_set_velocity", "dglm_set_time", "dglm_get_time", "dglm_rhs", "dglm_step",
     "dglm_integrals", "dglm_sync", "dglm_set_timing", "dglm_get_timing",
+    "dglm_set_stage_kernel", "dglm_get_stage_kernel",
 ]
 
 _lib = None

@@ -20,6 +20,7 @@
 #include "host/mesh.h"
 #include "host/refelem.h"
 #include "kernels/dg_fp64.hpp"
+#include "kernels/dg_mfma.hpp"
 
 #define DGLM_NTIMERS 3  // wave speed, dt finalize, stage
 
@@ -62,6 +63,9 @@ struct dglm_ctx {
   DevScalars *scal = nullptr;
   int cur = 0;
   int nblk_ws = 0;
+  int variant = DGLM_STAGE_AUTO;  // requested stage kernel
+  double *afrag = nullptr;        // operator A-fragments for the MFMA kernel
+  int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
   std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
   // Optional per-kernel timing with HIP events on the context's stream.
   bool timing = false;
@@ -216,8 +220,33 @@ void launch_wavespeed(dglm_ctx *c) {
                      c->vel, c->geo, c->fmx, c->maxv, c->partial, c->p.gamma);
 }
 
+// True when the MFMA stage kernel is built for this order.
+inline bool mfma_available(int N) { return N >= 1 && N <= 5; }
+
+bool use_mfma(const dglm_ctx *c) {
+  if (!mfma_available(c->p.N) || !c->afrag) return false;
+  return c->variant != DGLM_STAGE_SIMPLE;
+}
+
 template <int N, int PHYS, int MODE>
 void launch_stage(dglm_ctx *c, const StageArgs &a) {
+  if constexpr (N <= 5) {
+    if (use_mfma(c)) {
+      using S = MfmaShape<N>;
+      const int nbatch = (c->K + 15) / 16;
+      if (c->mfma_grid == 0) {
+        int dev = 0, cus = 0, occ = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_mfma_kernel<N, PHYS, 0>, S::NT, 0);
+        c->mfma_grid = (cus > 0 ? cus : 256) * (occ > 0 ? occ : 1);
+      }
+      const int grid = nbatch < c->mfma_grid ? nbatch : c->mfma_grid;
+      hipLaunchKernelGGL((stage_mfma_kernel<N, PHYS, MODE>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->afrag,
+                         nbatch);
+      return;
+    }
+  }
   constexpr int EB = eb_simple<N, PHYS>();
   int nblk = (c->K + EB - 1) / EB;
   hipLaunchKernelGGL((stage_simple_kernel<N, PHYS, MODE>), dim3(nblk), dim3(256), 0, c->stream, a);
@@ -285,7 +314,7 @@ void free_ctx(dglm_ctx *c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
-                  c->conn, c->fmx, c->scal};
+                  c->conn, c->fmx, c->scal, c->afrag};
   for (void *q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -414,6 +443,14 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   for (int q = 0; q < NFF; q++)
     for (int i = 0; i < Np; i++) op[(size_t)(3 * Np + q) * Np + i] = -at(m->LIFT, Np, NFF, i, q);
 
+  // A-fragments of the stacked operator for the MFMA stage kernel.
+  std::vector<double> afr;
+  if (mfma_available(p->N)) {
+    const int RT = (Np + 15) / 16, KS = (KD + 3) / 4;
+    afr.resize((size_t)RT * KS * 64);
+    arrange_a_fragments(Np, KD, RT, KS, op.data(), afr.data());
+  }
+
   dglm_ctx *c = new dglm_ctx();
   c->p = *p;
   c->K = K;
@@ -437,7 +474,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (rc = dev_alloc(&c->op, op.size(), "op")) || (rc = dev_alloc(&c->maxv, (size_t)K * 4, "maxv")) ||
       (rc = dev_alloc(&c->partial, (size_t)c->nblk_ws, "partial")) ||
       (rc = dev_alloc(&c->conn, conn.size(), "conn")) || (rc = dev_alloc(&c->fmx, fmx.size(), "fmx")) ||
-      (rc = dev_alloc(&c->scal, 1, "scalars"))) {
+      (rc = dev_alloc(&c->scal, 1, "scalars")) ||
+      (!afr.empty() && (rc = dev_alloc(&c->afrag, afr.size(), "A-fragments")))) {
     free_ctx(c);
     return rc;
   }
@@ -457,6 +495,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       hipMemcpy(c->op, op.data(), op.size() * sizeof(double), hipMemcpyHostToDevice),
       hipMemcpy(c->conn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice),
       hipMemcpy(c->fmx, fmx.data(), fmx.size() * sizeof(int), hipMemcpyHostToDevice),
+      c->afrag ? hipMemcpy(c->afrag, afr.data(), afr.size() * sizeof(double), hipMemcpyHostToDevice) : hipSuccess,
       hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
       hipMemset(c->u[0], 0, nst * sizeof(double)),
       hipMemset(c->u[1], 0, nst * sizeof(double)),
@@ -722,6 +761,22 @@ extern "C" int dglm_integrals(dglm_ctx *c, const double *J, const double *M, dou
     }
     out[v] = s;
   }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: null context");
+  if (variant != DGLM_STAGE_AUTO && variant != DGLM_STAGE_SIMPLE && variant != DGLM_STAGE_MFMA)
+    return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: unknown variant");
+  if (variant == DGLM_STAGE_MFMA && !mfma_available(c->p.N))
+    return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: no MFMA kernel for this order");
+  c->variant = variant;
+  return DGLM_OK;
+}
+
+extern "C" int dglm_get_stage_kernel(dglm_ctx *c, int *variant) {
+  if (!c || !variant) return fail(DGLM_ERR_ARG, "dglm_get_stage_kernel: null argument");
+  *variant = use_mfma(c) ? DGLM_STAGE_MFMA : DGLM_STAGE_SIMPLE;
   return DGLM_OK;
 }
 

@@ -126,6 +126,16 @@ class Solver:
         A.check(A.lib().dglm_step(self._ctx, nsteps, None))
         return None
 
+    def set_stage_kernel(self, variant):
+        """'auto', 'simple' (LDS-staged FP64) or 'mfma' (FP64 MFMA, N <= 5)."""
+        code = {"auto": A.DGLM_STAGE_AUTO, "simple": A.DGLM_STAGE_SIMPLE, "mfma": A.DGLM_STAGE_MFMA}[variant]
+        A.check(A.lib().dglm_set_stage_kernel(self._ctx, code))
+
+    def stage_kernel(self):
+        v = C.c_int()
+        A.check(A.lib().dglm_get_stage_kernel(self._ctx, C.byref(v)))
+        return {A.DGLM_STAGE_SIMPLE: "simple", A.DGLM_STAGE_MFMA: "mfma"}[v.value]
+
     def set_timing(self, on=True):
         A.check(A.lib().dglm_set_timing(self._ctx, 1 if on else 0))
 

@@ -167,6 +167,14 @@ int dglm_sync(dglm_ctx *ctx);
  * classes 0 = wave speed, 1 = dt finalize, 2 = fused stage, the summed
  * milliseconds and launch counts (synchronises the stream). */
 int dglm_set_timing(dglm_ctx *ctx, int on);
+
+/* Stage-kernel selection (the reference picks its GPU variant by file prefix,
+ * *_nopad_ / *_pad_, dglm_3d_launcher.py:176-205).  AUTO uses the FP64 MFMA
+ * kernel where it is built (N <= 5) and the LDS-staged kernel otherwise;
+ * dglm_get_stage_kernel reports the kernel that runs. */
+enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2 };
+int dglm_set_stage_kernel(dglm_ctx *ctx, int variant);
+int dglm_get_stage_kernel(dglm_ctx *ctx, int *variant);
 int dglm_get_timing(dglm_ctx *ctx, double *ms, long *count, int n);
 
 #ifdef __cplusplus
