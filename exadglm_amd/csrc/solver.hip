@@ -64,7 +64,8 @@ struct dglm_ctx {
   int cur = 0;
   int nblk_ws = 0;
   int variant = DGLM_STAGE_AUTO;  // requested stage kernel
-  double *afrag = nullptr;        // operator A-fragments for the MFMA kernel
+  void *afrag = nullptr;          // operator A-fragments for the MFMA kernel (FP64 or FP32)
+  double *rowsum = nullptr;       // FP64 block row sums of the operator (mixed precision)
   int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
   std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
   // Optional per-kernel timing with HIP events on the context's stream.
@@ -221,31 +222,39 @@ void launch_wavespeed(dglm_ctx *c) {
 }
 
 // True when the MFMA stage kernel is built for this order.
-inline bool mfma_available(int N) { return N >= 1 && N <= 5; }
+inline bool mfma_available(int N) { return N >= 1 && N <= 6; }
 
+// The mixed-precision contraction exists only as an MFMA kernel; FP64 can use
+// either kernel.
 bool use_mfma(const dglm_ctx *c) {
   if (!mfma_available(c->p.N) || !c->afrag) return false;
-  return c->variant != DGLM_STAGE_SIMPLE;
+  return c->p.precision != DGLM_FP64 || c->variant != DGLM_STAGE_SIMPLE;
+}
+
+template <int N, int PHYS, int MODE, int PREC>
+void launch_mfma(dglm_ctx *c, const StageArgs &a) {
+  using S = MfmaShape<N>;
+  const int nbatch = (c->K + 15) / 16;
+  if (c->mfma_grid == 0) {
+    int dev = 0, cus = 0, occ = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_mfma_kernel<N, PHYS, 0, PREC>, S::NT, 0);
+    c->mfma_grid = (cus > 0 ? cus : 256) * (occ > 0 ? occ : 1);
+  }
+  const int grid = nbatch < c->mfma_grid ? nbatch : c->mfma_grid;
+  hipLaunchKernelGGL((stage_mfma_kernel<N, PHYS, MODE, PREC>), dim3(grid), dim3(S::NT), 0, c->stream, a,
+                     (const void *)c->afrag, c->rowsum, nbatch);
 }
 
 template <int N, int PHYS, int MODE>
 void launch_stage(dglm_ctx *c, const StageArgs &a) {
-  if constexpr (N <= 5) {
-    if (use_mfma(c)) {
-      using S = MfmaShape<N>;
-      const int nbatch = (c->K + 15) / 16;
-      if (c->mfma_grid == 0) {
-        int dev = 0, cus = 0, occ = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_mfma_kernel<N, PHYS, 0>, S::NT, 0);
-        c->mfma_grid = (cus > 0 ? cus : 256) * (occ > 0 ? occ : 1);
-      }
-      const int grid = nbatch < c->mfma_grid ? nbatch : c->mfma_grid;
-      hipLaunchKernelGGL((stage_mfma_kernel<N, PHYS, MODE>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->afrag,
-                         nbatch);
-      return;
-    }
+  if (use_mfma(c)) {
+    if (c->p.precision == DGLM_FP32)
+      launch_mfma<N, PHYS, MODE, PREC_F32>(c, a);
+    else
+      launch_mfma<N, PHYS, MODE, PREC_F64>(c, a);
+    return;
   }
   constexpr int EB = eb_simple<N, PHYS>();
   int nblk = (c->K + EB - 1) / EB;
@@ -314,7 +323,7 @@ void free_ctx(dglm_ctx *c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
-                  c->conn, c->fmx, c->scal, c->afrag};
+                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum};
   for (void *q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
@@ -365,7 +374,12 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   *out = nullptr;
   if (p->physics != DGLM_ADVECTION && p->physics != DGLM_EULER)
     return fail(DGLM_ERR_ARG, "dglm_create: physics must be DGLM_ADVECTION or DGLM_EULER");
-  if (p->precision != DGLM_FP64) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: precision mode not built");
+  if (p->precision == DGLM_BF16X3)
+    return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: the split-BF16 contraction is not built");
+  if (p->precision != DGLM_FP64 && p->precision != DGLM_FP32)
+    return fail(DGLM_ERR_ARG, "dglm_create: unknown precision mode");
+  if (p->precision == DGLM_FP32 && (p->N < 1 || p->N > 6))
+    return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: mixed precision is built for N in [1, 6]");
   if (p->flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: only the LM flux is built");
   if (p->N < 1 || p->N > 6) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: N must be in [1, 6]");
   if (m->K <= 0) return fail(DGLM_ERR_ARG, "dglm_create: K must be positive");
@@ -445,11 +459,22 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
 
   // A-fragments of the stacked operator for the MFMA stage kernel.
   std::vector<double> afr;
+  std::vector<float> afr32;
+  std::vector<double> rsum;
   if (mfma_available(p->N)) {
     const int RT = (Np + 15) / 16, KS = (KD + 3) / 4;
-    afr.resize((size_t)RT * KS * 64);
-    arrange_a_fragments(Np, KD, RT, KS, op.data(), afr.data());
+    if (p->precision == DGLM_FP32) {
+      afr32.resize((size_t)RT * KS * 64);
+      arrange_a_fragments(Np, KD, RT, KS, op.data(), afr32.data());
+      rsum.resize((size_t)Np * NBLK);
+      block_row_sums(Np, Nfp, op.data(), rsum.data());
+    } else {
+      afr.resize((size_t)RT * KS * 64);
+      arrange_a_fragments(Np, KD, RT, KS, op.data(), afr.data());
+    }
   }
+  const size_t afrag_bytes = afr.size() * sizeof(double) + afr32.size() * sizeof(float);
+  const void *afrag_src = afr.empty() ? (const void *)afr32.data() : (const void *)afr.data();
 
   dglm_ctx *c = new dglm_ctx();
   c->p = *p;
@@ -475,7 +500,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (rc = dev_alloc(&c->partial, (size_t)c->nblk_ws, "partial")) ||
       (rc = dev_alloc(&c->conn, conn.size(), "conn")) || (rc = dev_alloc(&c->fmx, fmx.size(), "fmx")) ||
       (rc = dev_alloc(&c->scal, 1, "scalars")) ||
-      (!afr.empty() && (rc = dev_alloc(&c->afrag, afr.size(), "A-fragments")))) {
+      (afrag_bytes && (rc = dev_alloc((char **)&c->afrag, afrag_bytes, "A-fragments"))) ||
+      (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "row sums")))) {
     free_ctx(c);
     return rc;
   }
@@ -495,7 +521,9 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       hipMemcpy(c->op, op.data(), op.size() * sizeof(double), hipMemcpyHostToDevice),
       hipMemcpy(c->conn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice),
       hipMemcpy(c->fmx, fmx.data(), fmx.size() * sizeof(int), hipMemcpyHostToDevice),
-      c->afrag ? hipMemcpy(c->afrag, afr.data(), afr.size() * sizeof(double), hipMemcpyHostToDevice) : hipSuccess,
+      c->afrag ? hipMemcpy(c->afrag, afrag_src, afrag_bytes, hipMemcpyHostToDevice) : hipSuccess,
+      c->rowsum ? hipMemcpy(c->rowsum, rsum.data(), rsum.size() * sizeof(double), hipMemcpyHostToDevice)
+                : hipSuccess,
       hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
       hipMemset(c->u[0], 0, nst * sizeof(double)),
       hipMemset(c->u[1], 0, nst * sizeof(double)),
@@ -770,6 +798,8 @@ extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
     return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: unknown variant");
   if (variant == DGLM_STAGE_MFMA && !mfma_available(c->p.N))
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: no MFMA kernel for this order");
+  if (variant == DGLM_STAGE_SIMPLE && c->p.precision != DGLM_FP64)
+    return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: the LDS-staged kernel is FP64 only");
   c->variant = variant;
   return DGLM_OK;
 }
