@@ -26,6 +26,15 @@ def _say(msg):
     print(msg, flush=True)
 
 
+def euler_errors(u, uo):
+    """Relative L2 per conserved variable; rho*w, which is zero in the exact
+    z-extruded vortex (its own norm is discretisation noise), is measured
+    against the norm of the whole momentum."""
+    errs = rel_err_per_var(u, uo)
+    errs[3] = float(np.linalg.norm(u[3] - uo[3]) / np.linalg.norm(uo[1:4]))
+    return errs
+
+
 def _variants(N):
     return ["simple", "mfma"]
 
@@ -85,8 +94,8 @@ def test_euler_vortex_steps(gpu, N):
         u = s.get_state()
     uo, dto, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 40)
     assert rel_err(dts, dto) < 1e-13
-    for e in rel_err_per_var(u, uo):
-        assert e < 1e-12
+    errs = euler_errors(u, uo)
+    assert max(errs) < 1e-12, errs
 
 
 def test_advection_velocity_field(gpu):
@@ -126,8 +135,8 @@ def test_c2_full_size_properties(gpu):
         u23b = s.get_state()
     _say("  oracle 3 steps")
     uo, _, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 3)
-    for e in rel_err_per_var(u3, uo):
-        assert e < 1e-12
+    errs = euler_errors(u3, uo)
+    assert max(errs) < 1e-12, errs
     for v in (0, 1, 2, 4):
         assert abs(I1[v] - I0[v]) <= 1e-12 * abs(I0[v])
     assert np.array_equal(u23, u23b)
@@ -295,7 +304,5 @@ def test_mixed_precision_vortex(gpu, N):
         s.step(40)
         u = s.get_state()
     uo, _, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 40)
-    errs = rel_err_per_var(u, uo)
-    for v in (0, 1, 2, 4):
-        assert errs[v] < 5e-8, errs
-    assert np.linalg.norm(u[3] - uo[3]) / np.linalg.norm(uo[1:4]) < 5e-8
+    errs = euler_errors(u, uo)
+    assert max(errs) < 5e-8, errs
