@@ -73,7 +73,8 @@ EXPORTS = [
     "dglm_destroy", "dglm_sizes", "dglm_set_state", "dglm_get_state", "dglm_set_state_device",
     "dglm_get_state_device", "dglm_set_velocity", "dglm_set_time", "dglm_get_time", "dglm_rhs", "dglm_step",
     "dglm_integrals", "dglm_sync", "dglm_set_timing", "dglm_get_timing",
-    "dglm_set_stage_kernel", "dglm_get_stage_kernel",
+    "dglm_set_stage_kernel", "dglm_get_stage_kernel", "dglm_adjoint_rhs", "dglm_adjoint_record",
+    "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_tape_bytes",
 ]
 
 _lib = None

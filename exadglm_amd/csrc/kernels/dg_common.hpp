@@ -25,6 +25,8 @@ struct DevScalars {
   int tstep;       // completed steps after the current one
   int active;      // 0 once t_next >= tmax: stage kernels become no-ops
   int bad;         // non-finite dt seen
+  int fstar;       // face attaining max(lambda * Fscale) (smallest index on ties)
+  int clipped;     // 1 if dt of the current step was clipped to end at tmax
   int pad;
 };
 

@@ -58,11 +58,12 @@ __global__ __launch_bounds__(256) void wavespeed_kernel(int K, const double *__r
                                                         const double *__restrict__ vel,
                                                         const double *__restrict__ geo, const int *__restrict__ fmx,
                                                         double *__restrict__ maxv, double *__restrict__ partial,
-                                                        double gamma) {
+                                                        int *__restrict__ partial_idx, double gamma) {
   using O = Ord<N>;
   constexpr int Np = O::Np, Nfp = O::Nfp;
   const size_t KNp = (size_t)K * Np;
   __shared__ double red[256];
+  __shared__ int red_i[256];
   const int fid = blockIdx.x * 256 + threadIdx.x;
   double lfs = 0.0;
   if (fid < K * 4) {
@@ -90,36 +91,69 @@ __global__ __launch_bounds__(256) void wavespeed_kernel(int K, const double *__r
     maxv[fid] = lam;
     lfs = lam * geo[(size_t)e * GEO_STRIDE + 9 + f * 4 + 3];
   }
+  // Block maximum of lambda*Fscale with its face index (smallest on ties).
   red[threadIdx.x] = lfs;
+  red_i[threadIdx.x] = fid;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    if (threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      if (o > red[threadIdx.x] || (o == red[threadIdx.x] && red_i[threadIdx.x + s] < red_i[threadIdx.x])) {
+        red[threadIdx.x] = o;
+        red_i[threadIdx.x] = red_i[threadIdx.x + s];
+      }
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = red[0];
+    partial_idx[blockIdx.x] = red_i[0];
+  }
 }
 
 // dt = CFL/(2N+1) / max(lambda*Fscale), clipped so the run ends exactly at
 // tmax (euler_time.cpp:49-57).  One block.
-__global__ __launch_bounds__(1024) void finalize_dt_kernel(const double *__restrict__ partial, int n,
+__global__ __launch_bounds__(1024) void finalize_dt_kernel(const double *__restrict__ partial,
+                                                            const int *__restrict__ partial_idx, int n,
                                                             DevScalars *s, double cfl_fac, double tmax,
-                                                            double *dt_hist, int hist_idx) {
+                                                            double *dt_hist, int *fstar_hist, int *clip_hist,
+                                                            int hist_idx) {
   __shared__ double red[1024];
+  __shared__ int red_i[1024];
   double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, partial[i]);
+  int mi = 0x7fffffff;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const double o = partial[i];
+    if (o > m || (o == m && partial_idx[i] < mi)) {
+      m = o;
+      mi = partial_idx[i];
+    }
+  }
   red[threadIdx.x] = m;
+  red_i[threadIdx.x] = mi;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    if (threadIdx.x < w) {
+      const double o = red[threadIdx.x + w];
+      if (o > red[threadIdx.x] || (o == red[threadIdx.x] && red_i[threadIdx.x + w] < red_i[threadIdx.x])) {
+        red[threadIdx.x] = o;
+        red_i[threadIdx.x] = red_i[threadIdx.x + w];
+      }
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const double maxfs = red[0];
     const double t = s->t_next;
     s->maxfs = maxfs;
+    s->fstar = red_i[0];
+    s->clipped = 0;
     if (t < tmax) {
       double dt = cfl_fac * (1.0 / maxfs);
-      if (t + dt > tmax) dt = tmax - t;
+      if (t + dt > tmax) {
+        dt = tmax - t;
+        s->clipped = 1;
+      }
       s->t = t;
       s->dt = dt;
       s->t_next = t + dt;
@@ -131,6 +165,8 @@ __global__ __launch_bounds__(1024) void finalize_dt_kernel(const double *__restr
       s->dt = 0.0;
     }
     if (dt_hist) dt_hist[hist_idx] = s->active ? s->dt : 0.0;
+    if (fstar_hist) fstar_hist[hist_idx] = s->fstar;
+    if (clip_hist) clip_hist[hist_idx] = s->clipped;
   }
 }
 

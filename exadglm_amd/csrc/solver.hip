@@ -21,6 +21,7 @@
 #include "host/refelem.h"
 #include "kernels/dg_fp64.hpp"
 #include "kernels/dg_mfma.hpp"
+#include "kernels/dg_adjoint.hpp"
 
 #define DGLM_NTIMERS 3  // wave speed, dt finalize, stage
 
@@ -59,7 +60,7 @@ struct dglm_ctx {
   double *k = nullptr, *rhs = nullptr, *vel = nullptr, *geo = nullptr, *op = nullptr, *maxv = nullptr,
          *partial = nullptr, *dt_hist = nullptr;
   int dt_hist_cap = 0;
-  int *conn = nullptr, *fmx = nullptr;
+  int *conn = nullptr, *fmx = nullptr, *partial_idx = nullptr;
   DevScalars *scal = nullptr;
   int cur = 0;
   int nblk_ws = 0;
@@ -67,6 +68,8 @@ struct dglm_ctx {
   void *afrag = nullptr;          // operator A-fragments for the MFMA kernel (FP64 or FP32)
   double *rowsum = nullptr;       // FP64 block row sums of the operator (mixed precision)
   int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
+  struct dglm_adj_tape *tape = nullptr;  // adjoint buffers and recorded trajectory
+  bool adj_exact = true;          // include the tau(u) and dt(u) sensitivities (Euler)
   std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
   // Optional per-kernel timing with HIP events on the context's stream.
   bool timing = false;
@@ -216,9 +219,9 @@ extern "C" int dglm_mesh_spatial_h(int K, const int *EtoV, const double *VXYZ, d
 namespace {
 
 template <int N, int PHYS>
-void launch_wavespeed(dglm_ctx *c) {
-  hipLaunchKernelGGL((wavespeed_kernel<N, PHYS>), dim3(c->nblk_ws), dim3(256), 0, c->stream, c->K, c->u[c->cur],
-                     c->vel, c->geo, c->fmx, c->maxv, c->partial, c->p.gamma);
+void launch_wavespeed(dglm_ctx *c, const double *u, double *maxv) {
+  hipLaunchKernelGGL((wavespeed_kernel<N, PHYS>), dim3(c->nblk_ws), dim3(256), 0, c->stream, c->K, u, c->vel,
+                     c->geo, c->fmx, maxv, c->partial, c->partial_idx, c->p.gamma);
 }
 
 // True when the MFMA stage kernel is built for this order.
@@ -261,7 +264,7 @@ void launch_stage(dglm_ctx *c, const StageArgs &a) {
   hipLaunchKernelGGL((stage_simple_kernel<N, PHYS, MODE>), dim3(nblk), dim3(256), 0, c->stream, a);
 }
 
-typedef void (*ws_fn)(dglm_ctx *);
+typedef void (*ws_fn)(dglm_ctx *, const double *, double *);
 typedef void (*st_fn)(dglm_ctx *, const StageArgs &);
 
 template <int PHYS>
@@ -311,6 +314,8 @@ StageArgs stage_args(dglm_ctx *c, int s) {
   return a;
 }
 
+void free_tape(struct dglm_adj_tape *T);
+
 template <typename T>
 int dev_alloc(T **p, size_t n, const char *what) {
   hipError_t e = hipMalloc((void **)p, n * sizeof(T) + 16);
@@ -323,9 +328,10 @@ void free_ctx(dglm_ctx *c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
-                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum};
+                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx};
   for (void *q : ptrs)
     if (q) hipFree(q);
+  free_tape(c->tape);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -498,6 +504,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (rc = dev_alloc(&c->k, nst, "k")) || (rc = dev_alloc(&c->geo, geo.size(), "geo")) ||
       (rc = dev_alloc(&c->op, op.size(), "op")) || (rc = dev_alloc(&c->maxv, (size_t)K * 4, "maxv")) ||
       (rc = dev_alloc(&c->partial, (size_t)c->nblk_ws, "partial")) ||
+      (rc = dev_alloc(&c->partial_idx, (size_t)c->nblk_ws, "partial index")) ||
       (rc = dev_alloc(&c->conn, conn.size(), "conn")) || (rc = dev_alloc(&c->fmx, fmx.size(), "fmx")) ||
       (rc = dev_alloc(&c->scal, 1, "scalars")) ||
       (afrag_bytes && (rc = dev_alloc((char **)&c->afrag, afrag_bytes, "A-fragments"))) ||
@@ -709,7 +716,7 @@ extern "C" int dglm_rhs(dglm_ctx *c, const double *u, double *rhs, double *maxv,
     if (rc) return rc;
   }
   HIP_TRY(hipMemcpyAsync(c->u[c->cur], u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ws(c);
+  ws(c, c->u[c->cur], c->maxv);
   StageArgs a = stage_args(c, -1);
   st1(c, a);
   HIP_TRY(hipGetLastError());
@@ -745,11 +752,12 @@ extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
   const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
   for (int st = 0; st < nsteps; st++) {
     int tk = timer_start(c, 0);
-    ws(c);
+    ws(c, c->u[c->cur], c->maxv);
     timer_stop(c, tk);
     tk = timer_start(c, 1);
-    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->nblk_ws, c->scal,
-                       cfl_fac, c->p.tmax, dt_hist ? c->dt_hist : nullptr, st);
+    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
+                       c->nblk_ws, c->scal, cfl_fac, c->p.tmax, dt_hist ? c->dt_hist : nullptr, (int *)nullptr,
+                       (int *)nullptr, st);
     timer_stop(c, tk);
     for (int s = 0; s < 5; s++) {
       StageArgs a = stage_args(c, s);
@@ -837,5 +845,348 @@ extern "C" int dglm_sync(dglm_ctx *c) {
   if (!c) return fail(DGLM_ERR_ARG, "dglm_sync: null context");
   HIP_TRY(hipSetDevice(c->dev));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint: RHS adjoint, trajectory recording with checkpoints, and
+// the reverse sweep (see kernels/dg_adjoint.hpp for the per-stage algebra).
+//
+// Tape layout (all in HBM):
+//   ckpt   states at the start of steps 0, E, 2E, ...       (ceil(n/E) states)
+//   seg    the 5E+1 stage states of one segment of E steps, recomputed from
+//          its checkpoint during the reverse sweep
+//   lam    wave speeds per face of each step of the segment (tau)
+//   dt, fstar, clip   per step: dt, argmax face of lambda*Fscale, clip flag
+// With E ~ sqrt(n) this holds ~sqrt(n) + 5 sqrt(n) states, and the forward is
+// recomputed once.
+// ---------------------------------------------------------------------------
+namespace {
+
+typedef void (*adj_fn)(dglm_ctx *, const AdjArgs &);
+typedef void (*lam_fn)(dglm_ctx *, const double *, const double *, int);
+
+template <int N, int PHYS>
+void launch_adj_a(dglm_ctx *c, const AdjArgs &a) {
+  constexpr int EB = eb_adjoint<N, PHYS>();
+  hipLaunchKernelGGL((adjoint_pass_a<N, PHYS>), dim3((c->K + EB - 1) / EB), dim3(256), 0, c->stream, a);
+}
+
+template <int N, int PHYS>
+void launch_adj_b(dglm_ctx *c, const AdjArgs &a) {
+  const int n = c->K * Ord<N>::Np;
+  hipLaunchKernelGGL((adjoint_pass_b<N, PHYS>), dim3((n + 255) / 256), dim3(256), 0, c->stream, a);
+}
+
+template <int N>
+void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step);
+
+template <int N, int PHYS>
+int adj_blocks() {
+  constexpr int EB = eb_adjoint<N, PHYS>();
+  return EB;
+}
+
+template <int PHYS>
+bool pick_adj(int N, adj_fn &A, adj_fn &B, lam_fn &Lf, int &eb) {
+  switch (N) {
+#define DGLM_ACASE(n)                      \
+  case n:                                  \
+    A = launch_adj_a<n, PHYS>;             \
+    B = launch_adj_b<n, PHYS>;             \
+    Lf = launch_lam<n>;                    \
+    eb = adj_blocks<n, PHYS>();            \
+    return true;
+    DGLM_ACASE(1)
+    DGLM_ACASE(2)
+    DGLM_ACASE(3)
+    DGLM_ACASE(4)
+    DGLM_ACASE(5)
+    DGLM_ACASE(6)
+#undef DGLM_ACASE
+    default:
+      return false;
+  }
+}
+
+StageArgs make_stage_args(dglm_ctx *c, const double *u_in, double *u_out, double *k, const double *maxv,
+                          const DevScalars *scal, int s) {
+  StageArgs a = stage_args(c, s);
+  a.u_in = u_in;
+  a.u_out = u_out;
+  a.k = k;
+  a.maxv = maxv;
+  a.scal = scal;
+  return a;
+}
+
+}  // namespace
+
+struct dglm_adj_tape {
+  int nsteps = 0, every = 0, nck = 0, eb = 1, nblk_a = 0;
+  double *ckpt = nullptr, *ck_t = nullptr, *dt = nullptr, *seg = nullptr, *lam = nullptr, *kscr = nullptr;
+  int *fstar = nullptr, *clip = nullptr, *ck_step = nullptr;
+  DevScalars *scal2 = nullptr;
+  double *ubar = nullptr, *kbar = nullptr, *gsur = nullptr, *taubar = nullptr, *dtpart = nullptr, *tsc = nullptr;
+  double *rbar = nullptr;
+  int *slots = nullptr, *fpos = nullptr;
+  size_t bytes = 0;
+};
+
+namespace {
+
+template <int N>
+void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step) {
+  dglm_adj_tape *T = c->tape;
+  hipLaunchKernelGGL((lambda_adjoint_kernel<N>), dim3((c->K + 255) / 256), dim3(256), 0, c->stream, c->K, u, lam,
+                     T->taubar, c->conn, c->fmx, T->dt, T->fstar, step, T->tsc, c->p.tau_scale, c->p.gamma,
+                     T->ubar);
+}
+
+void free_tape(dglm_adj_tape *T) {
+  if (!T) return;
+  void *ptrs[] = {T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2,
+                  T->ubar, T->kbar, T->gsur, T->taubar, T->dtpart, T->tsc, T->rbar, T->slots, T->fpos};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  delete T;
+}
+
+// Buffers common to the RHS adjoint and the reverse sweep.
+int ensure_adjoint_buffers(dglm_ctx *c) {
+  if (!c->tape) c->tape = new dglm_adj_tape();
+  dglm_adj_tape *T = c->tape;
+  if (T->ubar) return DGLM_OK;
+  adj_fn A, B;
+  lam_fn L;
+  int eb = 1;
+  bool ok = c->p.physics == DGLM_EULER ? pick_adj<PHYS_EULER>(c->p.N, A, B, L, eb)
+                                       : pick_adj<PHYS_ADV>(c->p.N, A, B, L, eb);
+  if (!ok) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: order not built");
+  T->eb = eb;
+  T->nblk_a = (c->K + eb - 1) / eb;
+  const size_t nst = (size_t)c->nvar * c->K * c->Np;
+  // face-node slots of each volume node, and node positions in each face mask
+  RefOps ref;
+  reference_operators(c->p.N, ref);
+  const int Np = c->Np, Nfp = c->Nfp;
+  std::vector<int> slots((size_t)Np * 3, -1), fpos((size_t)4 * Np, -1);
+  for (int m = 0; m < 4 * Nfp; m++) {
+    const int node = ref.Fmask[m];
+    for (int q = 0; q < 3; q++)
+      if (slots[node * 3 + q] < 0) {
+        slots[node * 3 + q] = m;
+        break;
+      }
+    fpos[(m / Nfp) * Np + node] = m % Nfp;
+  }
+  int rc;
+  if ((rc = dev_alloc(&T->ubar, nst, "ubar")) || (rc = dev_alloc(&T->kbar, nst, "kbar")) ||
+      (rc = dev_alloc(&T->gsur, (size_t)c->nvar * c->K * c->NFF, "face adjoint")) ||
+      (rc = dev_alloc(&T->taubar, (size_t)c->K * 4, "taubar")) ||
+      (rc = dev_alloc(&T->dtpart, (size_t)T->nblk_a, "dt partials")) || (rc = dev_alloc(&T->tsc, 4, "t scalars")) ||
+      (rc = dev_alloc(&T->slots, slots.size(), "node slots")) || (rc = dev_alloc(&T->fpos, fpos.size(), "fpos")))
+    return rc;
+  HIP_TRY(hipMemcpy(T->slots, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(T->fpos, fpos.data(), fpos.size() * sizeof(int), hipMemcpyHostToDevice));
+  T->bytes += (2 * nst + (size_t)c->nvar * c->K * c->NFF + (size_t)c->K * 4 + T->nblk_a) * sizeof(double);
+  return DGLM_OK;
+}
+
+AdjArgs adj_args(dglm_ctx *c) {
+  dglm_adj_tape *T = c->tape;
+  AdjArgs a{};
+  a.K = c->K;
+  a.ubar = T->ubar;
+  a.kbar = T->kbar;
+  a.gsur = T->gsur;
+  a.taubar = T->taubar;
+  a.vel = c->vel;
+  a.geo = c->geo;
+  a.conn = c->conn;
+  a.fmx = c->fmx;
+  a.node_slots = T->slots;
+  a.fpos = T->fpos;
+  a.op = c->op;
+  a.gamma = c->p.gamma;
+  a.tau_scale = c->p.tau_scale;
+  return a;
+}
+
+}  // namespace
+
+extern "C" int dglm_adjoint_rhs(dglm_ctx *c, const double *u, const double *rbar, double *ubar, double *taubar) {
+  if (!c || !u || !rbar || !ubar) return fail(DGLM_ERR_ARG, "dglm_adjoint_rhs: null argument");
+  HIP_TRY(hipSetDevice(c->dev));
+  int rc = ensure_adjoint_buffers(c);
+  if (rc) return rc;
+  dglm_adj_tape *T = c->tape;
+  const size_t nst = (size_t)c->nvar * c->K * c->Np;
+  if (!T->rbar && (rc = dev_alloc(&T->rbar, nst, "rbar"))) return rc;
+  ws_fn ws;
+  st_fn st0, st1;
+  adj_fn A, B;
+  lam_fn L;
+  int eb;
+  if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_adjoint_rhs: order not built");
+  if (c->p.physics == DGLM_EULER) pick_adj<PHYS_EULER>(c->p.N, A, B, L, eb);
+  else pick_adj<PHYS_ADV>(c->p.N, A, B, L, eb);
+  HIP_TRY(hipMemcpyAsync(c->u[c->cur], u, nst * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(T->rbar, rbar, nst * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ws(c, c->u[c->cur], c->maxv);  // tau at u
+  AdjArgs a = adj_args(c);
+  a.mode = 1;
+  a.u = c->u[c->cur];
+  a.rbar = T->rbar;
+  a.maxv = c->maxv;
+  A(c, a);
+  B(c, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(ubar, T->ubar, nst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (taubar)
+    HIP_TRY(hipMemcpyAsync(taubar, T->taubar, (size_t)c->K * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}
+
+extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
+  if (!c || nsteps <= 0 || every <= 0) return fail(DGLM_ERR_ARG, "dglm_adjoint_record: bad arguments");
+  HIP_TRY(hipSetDevice(c->dev));
+  int rc = ensure_adjoint_buffers(c);
+  if (rc) return rc;
+  dglm_adj_tape *T = c->tape;
+  ws_fn ws;
+  st_fn st0, st1;
+  if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_adjoint_record: order not built");
+  const size_t nst = (size_t)c->nvar * c->K * c->Np;
+  const int nck = (nsteps + every - 1) / every;
+  // (re)allocate the recording buffers when the shape changes
+  if (T->nsteps != nsteps || T->every != every) {
+    void *old[] = {T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2};
+    for (void *q : old)
+      if (q) HIP_TRY(hipFree(q));
+    T->ckpt = T->ck_t = T->dt = T->seg = T->lam = T->kscr = nullptr;
+    T->fstar = T->clip = T->ck_step = nullptr;
+    T->scal2 = nullptr;
+    if ((rc = dev_alloc(&T->ckpt, nst * nck, "checkpoints")) || (rc = dev_alloc(&T->ck_t, (size_t)nck, "ckpt t")) ||
+        (rc = dev_alloc(&T->ck_step, (size_t)nck, "ckpt step")) || (rc = dev_alloc(&T->dt, (size_t)nsteps, "dt")) ||
+        (rc = dev_alloc(&T->fstar, (size_t)nsteps, "fstar")) || (rc = dev_alloc(&T->clip, (size_t)nsteps, "clip")) ||
+        (rc = dev_alloc(&T->seg, nst * (5 * (size_t)every + 1), "segment states")) ||
+        (rc = dev_alloc(&T->lam, (size_t)c->K * 4 * every, "segment wave speeds")) ||
+        (rc = dev_alloc(&T->kscr, nst, "k scratch")) || (rc = dev_alloc(&T->scal2, 1, "recompute scalars")))
+      return rc;
+    T->nsteps = nsteps;
+    T->every = every;
+    T->nck = nck;
+    T->bytes = (2 * nst + (size_t)c->nvar * c->K * c->NFF + (size_t)c->K * 4 + T->nblk_a) * sizeof(double) +
+               (nst * (nck + 5 * (size_t)every + 2) + (size_t)c->K * 4 * every) * sizeof(double);
+  }
+  const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0) {
+      const int q = n / every;
+      HIP_TRY(hipMemcpyAsync(T->ckpt + nst * q, c->u[c->cur], nst * sizeof(double), hipMemcpyDeviceToDevice,
+                             c->stream));
+      HIP_TRY(hipMemcpyAsync(T->ck_t + q, &c->scal->t_next, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(T->ck_step + q, &c->scal->tstep, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    }
+    ws(c, c->u[c->cur], c->maxv);
+    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
+                       c->nblk_ws, c->scal, cfl_fac, c->p.tmax, T->dt, T->fstar, T->clip, n);
+    for (int s = 0; s < 5; s++) {
+      StageArgs a = stage_args(c, s);
+      st0(c, a);
+      c->cur ^= 1;
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return DGLM_OK;
+}
+
+extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *ubar_0, int device_ptrs) {
+  if (!c || !ubar_T || !ubar_0) return fail(DGLM_ERR_ARG, "dglm_adjoint_backward: null argument");
+  dglm_adj_tape *T = c->tape;
+  if (!T || T->nsteps == 0) return fail(DGLM_ERR_STATE, "dglm_adjoint_backward: nothing recorded");
+  HIP_TRY(hipSetDevice(c->dev));
+  ws_fn ws;
+  st_fn st0, st1;
+  adj_fn A, B;
+  lam_fn L;
+  int eb;
+  if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_adjoint_backward: order not built");
+  if (c->p.physics == DGLM_EULER) pick_adj<PHYS_EULER>(c->p.N, A, B, L, eb);
+  else pick_adj<PHYS_ADV>(c->p.N, A, B, L, eb);
+  const size_t nst = (size_t)c->nvar * c->K * c->Np;
+  const bool exact = c->adj_exact && c->p.physics == DGLM_EULER;
+  const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
+  const hipMemcpyKind kin = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIP_TRY(hipMemcpyAsync(T->ubar, ubar_T, nst * sizeof(double), kin, c->stream));
+  HIP_TRY(hipMemsetAsync(T->kbar, 0, nst * sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(T->kscr, 0, nst * sizeof(double), c->stream));
+  HIP_TRY(hipMemsetAsync(T->tsc, 0, 4 * sizeof(double), c->stream));
+  for (int q = T->nck - 1; q >= 0; q--) {
+    const int n0 = q * T->every, n1 = std::min(n0 + T->every, T->nsteps);
+    // recompute the segment's stage states from its checkpoint
+    DevScalars s0{};
+    HIP_TRY(hipMemcpyAsync(T->seg, T->ckpt + nst * q, nst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(T->scal2, 0, sizeof(DevScalars), c->stream));
+    HIP_TRY(hipMemcpyAsync(&T->scal2->t_next, T->ck_t + q, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    (void)s0;
+    for (int n = n0; n < n1; n++) {
+      double *base = T->seg + nst * 5 * (size_t)(n - n0);
+      double *lam = T->lam + (size_t)c->K * 4 * (n - n0);
+      ws(c, base, lam);
+      hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
+                         c->nblk_ws, T->scal2, cfl_fac, c->p.tmax, (double *)nullptr, (int *)nullptr, (int *)nullptr,
+                         0);
+      for (int s = 0; s < 5; s++) st0(c, make_stage_args(c, base + nst * s, base + nst * (s + 1), T->kscr, lam, T->scal2, s));
+    }
+    // reverse sweep over the segment
+    for (int n = n1 - 1; n >= n0; n--) {
+      double *base = T->seg + nst * 5 * (size_t)(n - n0);
+      const double *lam = T->lam + (size_t)c->K * 4 * (n - n0);
+      HIP_TRY(hipMemsetAsync(T->taubar, 0, (size_t)c->K * 4 * sizeof(double), c->stream));
+      HIP_TRY(hipMemsetAsync(T->dtpart, 0, (size_t)T->nblk_a * sizeof(double), c->stream));
+      for (int s = 4; s >= 0; s--) {
+        AdjArgs a = adj_args(c);
+        a.mode = 0;
+        a.u = base + nst * s;
+        a.maxv = lam;
+        a.dtp = T->dt + n;
+        a.a = RKA[s];
+        a.b = RKB[s];
+        if (exact) {
+          a.u_next = base + nst * (s + 1);
+          a.u_prev = s > 0 ? base + nst * (s - 1) : nullptr;
+          a.a_prev = s > 0 ? RKA[s - 1] : 0.0;
+          a.b_prev = s > 0 ? RKB[s - 1] : 1.0;
+          a.dtpart = T->dtpart;
+        }
+        A(c, a);
+        B(c, a);
+      }
+      if (exact) {
+        hipLaunchKernelGGL(adjoint_dt_reduce, dim3(1), dim3(256), 0, c->stream, T->dtpart, T->nblk_a, T->clip, n,
+                           T->tsc);
+        L(c, base, lam, n);
+      }
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  const hipMemcpyKind kout = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  HIP_TRY(hipMemcpyAsync(ubar_0, T->ubar, nst * sizeof(double), kout, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return DGLM_OK;
+}
+
+extern "C" int dglm_adjoint_set_exact(dglm_ctx *c, int on) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_adjoint_set_exact: null context");
+  c->adj_exact = on != 0;
+  return DGLM_OK;
+}
+
+extern "C" int dglm_adjoint_tape_bytes(dglm_ctx *c, double *bytes) {
+  if (!c || !bytes) return fail(DGLM_ERR_ARG, "dglm_adjoint_tape_bytes: null argument");
+  *bytes = c->tape ? (double)c->tape->bytes : 0.0;
   return DGLM_OK;
 }

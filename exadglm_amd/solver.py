@@ -155,3 +155,38 @@ class Solver:
         M = np.ascontiguousarray(M, dtype=np.float64)
         A.check(A.lib().dglm_integrals(self._ctx, A.dptr(J), A.dptr(M), A.dptr(out)))
         return out
+
+    # ---- discrete adjoint ----
+    def adjoint_rhs(self, u, rbar):
+        """(dR/du)^T rbar at u (tau held at u) and the per-face tau adjoint."""
+        u = self._state_arg(u)
+        rbar = self._state_arg(rbar)
+        ubar = np.empty(self.shape)
+        taubar = np.empty((self.K, 4))
+        A.check(A.lib().dglm_adjoint_rhs(self._ctx, A.dptr(u), A.dptr(rbar), A.dptr(ubar), A.dptr(taubar)))
+        return ubar, taubar
+
+    def adjoint_record(self, nsteps, every=None):
+        """Advance nsteps from the current state, checkpointing every `every`
+        steps (default ~sqrt(nsteps))."""
+        if every is None:
+            every = max(1, int(round(math.sqrt(nsteps))))
+        A.check(A.lib().dglm_adjoint_record(self._ctx, nsteps, every))
+
+    def adjoint_backward(self, ubar_T):
+        ubar_T = self._state_arg(ubar_T)
+        out = np.empty(self.shape)
+        A.check(A.lib().dglm_adjoint_backward(self._ctx, A.dptr(ubar_T), A.dptr(out), 0))
+        return out
+
+    def adjoint_backward_device(self, ubar_T_ptr, ubar_0_ptr):
+        A.check(A.lib().dglm_adjoint_backward(self._ctx, C.cast(C.c_void_p(ubar_T_ptr), A.PD),
+                                              C.cast(C.c_void_p(ubar_0_ptr), A.PD), 1))
+
+    def set_adjoint_exact(self, on=True):
+        A.check(A.lib().dglm_adjoint_set_exact(self._ctx, 1 if on else 0))
+
+    def tape_bytes(self):
+        b = C.c_double()
+        A.check(A.lib().dglm_adjoint_tape_bytes(self._ctx, C.byref(b)))
+        return b.value

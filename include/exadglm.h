@@ -177,6 +177,28 @@ int dglm_set_stage_kernel(dglm_ctx *ctx, int variant);
 int dglm_get_stage_kernel(dglm_ctx *ctx, int *variant);
 int dglm_get_timing(dglm_ctx *ctx, double *ms, long *count, int n);
 
+/* ---------------------------------------------------------------------------
+ * Discrete adjoint (no counterpart in the reference, which has no
+ * differentiation; it differentiates the path above exactly as implemented:
+ * update_rhs, calc_fluxLM, calc_fluxes and lserk_stage of euler_dglm.cpp /
+ * euler_time.cpp, including tau(u) and dt(u) through the argmax the forward
+ * took).  All adjoint arithmetic is FP64; a mixed-precision forward is
+ * recomputed with the same kernels during the reverse sweep.
+ * ------------------------------------------------------------------------- */
+/* ubar = (dR/du)^T rbar at state u with tau held at u's wave speeds; taubar
+ * ([K*4], optional) = dR/dtau^T rbar per face side.  Host arrays. */
+int dglm_adjoint_rhs(dglm_ctx *ctx, const double *u, const double *rbar, double *ubar, double *taubar);
+/* Advance nsteps from the current state like dglm_step, keeping a checkpoint
+ * every `every` steps (sqrt(nsteps) balances memory and recompute). */
+int dglm_adjoint_record(dglm_ctx *ctx, int nsteps, int every);
+/* Reverse sweep over the recorded steps: ubar_0 = (du_T/du_0)^T ubar_T.
+ * device_ptrs != 0: both pointers are device memory ([nvar][K*Np]). */
+int dglm_adjoint_backward(dglm_ctx *ctx, const double *ubar_T, double *ubar_0, int device_ptrs);
+/* on = 0 freezes tau and dt at their forward values (the exact gradient of
+ * the frozen-coefficient map); default 1 (exact; only Euler depends on it). */
+int dglm_adjoint_set_exact(dglm_ctx *ctx, int on);
+int dglm_adjoint_tape_bytes(dglm_ctx *ctx, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -257,6 +257,18 @@ int oracle_rhs(const oracle_problem *P, const double *u, double *rhs, double *ma
   return 0;
 }
 
+int oracle_rhs_tau(const oracle_problem *P, const double *u, const double *tau, double *rhs) {
+  work_t w;
+  if (work_alloc(P, &w)) {
+    work_free(&w);
+    return -1;
+  }
+  memcpy(w.tau, tau, (size_t)P->K * NFACE * sizeof(double));
+  rhs_core(P, &w, u, rhs);
+  work_free(&w);
+  return 0;
+}
+
 int oracle_run(const oracle_problem *P, double *u, double *k, int nsteps, double *t, int *tstep, double *dt_hist) {
   work_t w;
   const size_t n = (size_t)P->nvar * P->K * P->Np;
@@ -290,5 +302,90 @@ int oracle_run(const oracle_problem *P, double *u, double *k, int nsteps, double
   }
   work_free(&w);
   free(utmp);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * Tangent-linear RHS with tau frozen at u: drhs = dR/du(u) du.
+ * J_n(U) dU of the Euler flux n.F (derivative of the flux formulas above);
+ * pinned by central differences of oracle_rhs in tests/test_oracle.py.
+ * ---------------------------------------------------------------------- */
+static void euler_jv(const double U[5], double nx, double ny, double nz, const double d[5], double G, double out[5]) {
+  double inv = 1.0 / U[0];
+  double v[3] = {U[1] * inv, U[2] * inv, U[3] * inv}, n[3] = {nx, ny, nz};
+  double q2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  double p = (G - 1) * (U[4] - 0.5 * U[0] * q2);
+  double H = (U[4] + p) * inv;
+  double vn = n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+  double gm = G - 1;
+  double ndm = n[0] * d[1] + n[1] * d[2] + n[2] * d[3];
+  double vdm = v[0] * d[1] + v[1] * d[2] + v[2] * d[3];
+  out[0] = ndm;
+  for (int c = 0; c < 3; c++)
+    out[1 + c] = (-v[c] * vn + n[c] * gm * 0.5 * q2) * d[0] + v[c] * ndm + vn * d[1 + c] - n[c] * gm * vdm +
+                 n[c] * gm * d[4];
+  out[4] = vn * (gm * 0.5 * q2 - H) * d[0] + H * ndm - gm * vn * vdm + G * vn * d[4];
+}
+
+int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, double *drhs) {
+  const int K = P->K, Np = P->Np, Nfp = P->Nfp, NFF = NFACE * Nfp, nv = P->nvar;
+  const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
+  work_t w;
+  if (work_alloc(P, &w)) {
+    work_free(&w);
+    return -1;
+  }
+  calc_maxv(P, u, w.maxv, w.maxv_fs);
+  update_tau(P, w.maxv, w.tau);
+  /* dB volume -> w.fR/fS/fT, dB surface -> w.fLM */
+#pragma omp parallel for
+  for (int ei = 0; ei < K; ei++) {
+    double m[9] = {P->rx[ei], P->ry[ei], P->rz[ei], P->sx[ei], P->sy[ei], P->sz[ei], P->tx[ei], P->ty[ei], P->tz[ei]};
+    for (int j = 0; j < Np; j++) {
+      size_t idx = (size_t)ei * Np + j;
+      double JF[3][5];
+      if (nv == 5) {
+        double U[5], D[5];
+        for (int v = 0; v < 5; v++) {
+          U[v] = u[v * KNp + idx];
+          D[v] = du[v * KNp + idx];
+        }
+        for (int x = 0; x < 3; x++) euler_jv(U, x == 0, x == 1, x == 2, D, P->gamma, JF[x]);
+      } else {
+        JF[0][0] = P->vx[idx] * du[idx];
+        JF[1][0] = P->vy[idx] * du[idx];
+        JF[2][0] = P->vz[idx] * du[idx];
+      }
+      for (int v = 0; v < nv; v++) {
+        w.fR[v * KNp + idx] = m[0] * JF[0][v] + m[1] * JF[1][v] + m[2] * JF[2][v];
+        w.fS[v * KNp + idx] = m[3] * JF[0][v] + m[4] * JF[1][v] + m[5] * JF[2][v];
+        w.fT[v * KNp + idx] = m[6] * JF[0][v] + m[7] * JF[1][v] + m[8] * JF[2][v];
+      }
+    }
+  }
+#pragma omp parallel for
+  for (int fidx = 0; fidx < K * NFACE; fidx++) {
+    int ei = fidx / NFACE, fi = fidx % NFACE;
+    double nx = P->nx[fidx], ny = P->ny[fidx], nz = P->nz[fidx], fs = P->Fscale[fidx], tau = w.tau[fidx];
+    for (int j = 0; j < Nfp; j++) {
+      size_t idx = (size_t)fidx * Nfp + j;
+      size_t iM = (size_t)ei * Np + P->Fmask[fi * Nfp + j], iP = (size_t)P->vmapP[idx];
+      if (nv == 5) {
+        double ub[5], dub[5], out[5];
+        for (int v = 0; v < 5; v++) {
+          ub[v] = 0.5 * (u[v * KNp + iM] + u[v * KNp + iP]);
+          dub[v] = 0.5 * (du[v * KNp + iM] + du[v * KNp + iP]);
+        }
+        euler_jv(ub, nx, ny, nz, dub, P->gamma, out);
+        for (int v = 0; v < 5; v++) w.fLM[v * KNF + idx] = (out[v] - tau * (dub[v] - du[v * KNp + iM])) * fs;
+      } else {
+        double dub = 0.5 * (du[iM] + du[iP]);
+        double vn = nx * P->vx[iM] + ny * P->vy[iM] + nz * P->vz[iM];
+        w.fLM[idx] = (vn * dub - tau * (dub - du[iM])) * fs;
+      }
+    }
+  }
+  update_rhs(P, w.fR, w.fS, w.fT, w.fLM, drhs);
+  work_free(&w);
   return 0;
 }

@@ -25,6 +25,12 @@ typedef struct oracle_problem {
 /* One RHS at state u ([nvar][K*Np]); maxv, tau [K*4] optional. */
 int oracle_rhs(const oracle_problem *P, const double *u, double *rhs, double *maxv, double *tau);
 
+/* RHS at u with the face penalty tau [K*4] given instead of computed. */
+int oracle_rhs_tau(const oracle_problem *P, const double *u, const double *tau, double *rhs);
+
+/* Tangent-linear RHS with tau frozen at u: drhs = dR/du(u) du. */
+int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, double *drhs);
+
 /* nsteps of runRK on (u, k) in place; t and tstep advance; steps once t has
  * reached tmax are skipped (dt_hist entry 0). */
 int oracle_run(const oracle_problem *P, double *u, double *k, int nsteps, double *t, int *tstep, double *dt_hist);

@@ -166,3 +166,23 @@ class RefSolver:
     def run(self):
         """The reference main loop to TMAX; returns stdout and the dataout dir."""
         return self._call("run"), self.dataout
+
+
+def rhs_tl(oracle, u, du):
+    """Tangent-linear RHS of the oracle (tau frozen at u)."""
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(oracle.nvar, -1)
+    du = np.ascontiguousarray(du, dtype=np.float64).reshape(oracle.nvar, -1)
+    out = np.empty_like(u)
+    if lib().oracle_rhs_tl(C.byref(oracle.P), _d(u), _d(du), _d(out)) != 0:
+        raise MemoryError("oracle_rhs_tl allocation failed")
+    return out
+
+
+def rhs_tau(oracle, u, tau):
+    """Oracle RHS at u with a given face penalty tau [K, 4]."""
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(oracle.nvar, -1)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    out = np.empty_like(u)
+    if lib().oracle_rhs_tau(C.byref(oracle.P), _d(u), _d(tau), _d(out)) != 0:
+        raise MemoryError("oracle_rhs_tau allocation failed")
+    return out
