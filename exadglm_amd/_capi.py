@@ -24,6 +24,7 @@ DGLM_FP64 = 0
 DGLM_FP32 = 1
 DGLM_BF16X3 = 2
 DGLM_FLUX_LM = 0
+DGLM_FLUX_RUSANOV = 1
 DGLM_STAGE_AUTO = 0
 DGLM_STAGE_SIMPLE = 1
 DGLM_STAGE_MFMA = 2
@@ -74,7 +75,7 @@ EXPORTS = [
     "dglm_get_state_device", "dglm_set_velocity", "dglm_set_time", "dglm_get_time", "dglm_rhs", "dglm_step",
     "dglm_integrals", "dglm_sync", "dglm_set_timing", "dglm_get_timing",
     "dglm_set_stage_kernel", "dglm_get_stage_kernel", "dglm_adjoint_rhs", "dglm_adjoint_record",
-    "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_tape_bytes",
+    "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_tape_bytes", "dglm_set_graphs",
 ]
 
 _lib = None

@@ -50,6 +50,26 @@ __device__ inline void euler_flux(const double u[5], double g, double F[5], doub
   H[4] = ep * vz;
 }
 
+// Textbook Rusanov (local Lax-Friedrichs) normal flux of variable v:
+//   F*.n = (n.F(u-) + n.F(u+))/2 + lambda/2 (u- - u+),
+//   lambda = max(|v-.n| + c-, |v+.n| + c+)   (a flux option beside the LM flux).
+__device__ inline double euler_normal_flux(int v, const double U[5], const double *n, double gamma, double &lam) {
+  const double inv = 1.0 / U[0];
+  const double p = (gamma - 1.0) * (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * inv);
+  const double vn = (n[0] * U[1] + n[1] * U[2] + n[2] * U[3]) * inv;
+  lam = fabs(vn) + sqrt(fabs(gamma * p * inv));
+  if (v == 0) return vn * U[0];
+  if (v == 4) return (U[4] + p) * vn;
+  return U[v] * vn + n[v - 1] * p;
+}
+
+__device__ inline double euler_rusanov(int v, const double uM[5], const double uP[5], const double *n, double gamma) {
+  double lm, lp;
+  const double fm = euler_normal_flux(v, uM, n, gamma, lm);
+  const double fp = euler_normal_flux(v, uP, n, gamma, lp);
+  return 0.5 * (fm + fp) + 0.5 * fmax(lm, lp) * (uM[v] - uP[v]);
+}
+
 // --------------------------------------------------------------------------
 // Wave speed per face + block maximum of lambda*Fscale
 // --------------------------------------------------------------------------
@@ -170,6 +190,32 @@ __global__ __launch_bounds__(1024) void finalize_dt_kernel(const double *__restr
   }
 }
 
+// Conserved integrals sum_e J_e w^T u_e (w = column sums of M), FP64: one
+// thread per element, a fixed-order block reduction, partials per block.
+__global__ __launch_bounds__(256) void integrals_kernel(int K, int Np, int nvar, const double *__restrict__ u,
+                                                       const double *__restrict__ J,
+                                                       const double *__restrict__ w, double *partial) {
+  __shared__ double red[256];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const size_t KNp = (size_t)K * Np;
+  for (int v = 0; v < nvar; v++) {
+    double s = 0.0;
+    if (e < K) {
+      const double *ue = u + v * KNp + (size_t)e * Np;
+      for (int i = 0; i < Np; i++) s += w[i] * ue[i];
+      s *= J[e];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int q = 128; q > 0; q >>= 1) {
+      if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * nvar + v] = red[0];
+    __syncthreads();
+  }
+}
+
 // --------------------------------------------------------------------------
 // Fused stage kernel, straightforward LDS-staged form.
 //   MODE 0: k = a k + dt rhs; u_out = u_in + b k
@@ -190,6 +236,7 @@ struct StageArgs {
   const DevScalars *scal;
   double rka, rkb;
   double gamma, tau_scale;
+  int flux;  // 0: LM (reference), 1: Rusanov (upwind for advection)
 };
 
 template <int N, int PHYS>
@@ -280,19 +327,30 @@ __global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
     const double nx = g[0], ny = g[1], nz = g[2], fs = g[3];
     const double tau = s_tau[e * 4 + f];
     if (PHYS == PHYS_EULER) {
-      double uM[5], ub[5], F[5], G[5], H[5];
+      double uM[5], uP[5], ub[5], F[5], G[5], H[5];
       for (int v = 0; v < 5; v++) {
         uM[v] = s_u[v][mloc];
-        ub[v] = 0.5 * (uM[v] + a.u_in[v * KNp + pidx]);
+        uP[v] = a.u_in[v * KNp + pidx];
+        ub[v] = 0.5 * (uM[v] + uP[v]);
       }
-      euler_flux(ub, a.gamma, F, G, H);
-      for (int v = 0; v < 5; v++)
-        s_f[v][e * KD + 3 * Np + m] = ((nx * F[v] + ny * G[v] + nz * H[v]) - tau * (ub[v] - uM[v])) * fs;
+      if (a.flux == 1) {
+        for (int v = 0; v < 5; v++) s_f[v][e * KD + 3 * Np + m] = euler_rusanov(v, uM, uP, g, a.gamma) * fs;
+      } else {
+        euler_flux(ub, a.gamma, F, G, H);
+        for (int v = 0; v < 5; v++)
+          s_f[v][e * KD + 3 * Np + m] = ((nx * F[v] + ny * G[v] + nz * H[v]) - tau * (ub[v] - uM[v])) * fs;
+      }
     } else {
       const double uM = s_u[0][mloc];
-      const double ub = 0.5 * (uM + a.u_in[pidx]);
-      const double nf = nx * (s_vel[0][mloc] * ub) + ny * (s_vel[1][mloc] * ub) + nz * (s_vel[2][mloc] * ub);
-      s_f[0][e * KD + 3 * Np + m] = (nf - tau * (ub - uM)) * fs;
+      const double uP = a.u_in[pidx];
+      const double ub = 0.5 * (uM + uP);
+      const double vn = nx * s_vel[0][mloc] + ny * s_vel[1][mloc] + nz * s_vel[2][mloc];
+      if (a.flux == 1) {
+        s_f[0][e * KD + 3 * Np + m] = (vn * ub + 0.5 * fabs(vn) * (uM - uP)) * fs;
+      } else {
+        const double nf = nx * (s_vel[0][mloc] * ub) + ny * (s_vel[1][mloc] * ub) + nz * (s_vel[2][mloc] * ub);
+        s_f[0][e * KD + 3 * Np + m] = (nf - tau * (ub - uM)) * fs;
+      }
     }
   }
   __syncthreads();

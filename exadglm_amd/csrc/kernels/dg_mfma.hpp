@@ -181,6 +181,15 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
     const double tau = s_tau[e * 4 + f];
     if (PHYS == PHYS_EULER) {
       double ub[5];
+      if (a.flux == 1) {
+        double uM[5], uP[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+          uM[q] = s_u[q][mloc];
+          uP[q] = a.u_in[q * KNp + pidx];
+        }
+        return euler_rusanov(v, uM, uP, g, a.gamma) * g[3];
+      }
 #pragma unroll
       for (int q = 0; q < 5; q++) ub[q] = 0.5 * (s_u[q][mloc] + a.u_in[q * KNp + pidx]);
       const double inv = 1.0 / ub[0];
@@ -193,7 +202,12 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
       return (nF - tau * (ub[v] - s_u[v][mloc])) * g[3];
     } else {
       const double uM = s_u[0][mloc];
-      const double ub = 0.5 * (uM + a.u_in[pidx]);
+      const double uP = a.u_in[pidx];
+      const double ub = 0.5 * (uM + uP);
+      if (a.flux == 1) {
+        const double vn = g[0] * s_vel[0][mloc] + g[1] * s_vel[1][mloc] + g[2] * s_vel[2][mloc];
+        return (vn * ub + 0.5 * fabs(vn) * (uM - uP)) * g[3];
+      }
       const double nF = g[0] * (s_vel[0][mloc] * ub) + g[1] * (s_vel[1][mloc] * ub) + g[2] * (s_vel[2][mloc] * ub);
       return (nF - tau * (ub - uM)) * g[3];
     }

@@ -69,6 +69,9 @@ struct dglm_ctx {
   double *rowsum = nullptr;       // FP64 block row sums of the operator (mixed precision)
   int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
   struct dglm_adj_tape *tape = nullptr;  // adjoint buffers and recorded trajectory
+  bool use_graph = true;                 // replay two-step hipGraphs in dglm_step
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // keyed by the state buffer the step starts from
+  double *d_J = nullptr, *d_w = nullptr, *d_ipart = nullptr;  // conservation sums
   bool adj_exact = true;          // include the tau(u) and dt(u) sensitivities (Euler)
   std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
   // Optional per-kernel timing with HIP events on the context's stream.
@@ -332,6 +335,11 @@ void free_ctx(dglm_ctx *c) {
   for (void *q : ptrs)
     if (q) hipFree(q);
   free_tape(c->tape);
+  for (int g = 0; g < 2; g++)
+    if (c->graph[g]) hipGraphExecDestroy(c->graph[g]);
+  if (c->d_J) hipFree(c->d_J);
+  if (c->d_w) hipFree(c->d_w);
+  if (c->d_ipart) hipFree(c->d_ipart);
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -386,7 +394,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
     return fail(DGLM_ERR_ARG, "dglm_create: unknown precision mode");
   if (p->precision == DGLM_FP32 && (p->N < 1 || p->N > 6))
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: mixed precision is built for N in [1, 6]");
-  if (p->flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: only the LM flux is built");
+  if (p->flux != DGLM_FLUX_LM && p->flux != DGLM_FLUX_RUSANOV)
+    return fail(DGLM_ERR_ARG, "dglm_create: flux must be DGLM_FLUX_LM or DGLM_FLUX_RUSANOV");
   if (p->N < 1 || p->N > 6) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: N must be in [1, 6]");
   if (m->K <= 0) return fail(DGLM_ERR_ARG, "dglm_create: K must be positive");
   if (!(p->cfl > 0) || !(p->tau_scale >= 0) || !(p->gamma > 1.0) || std::isnan(p->tmax))
@@ -736,6 +745,55 @@ extern "C" int dglm_rhs(dglm_ctx *c, const double *u, double *rhs, double *maxv,
   return DGLM_OK;
 }
 
+namespace {
+// One LSRK4(5) step: wave speed, device-side dt, five fused stages.
+void enqueue_step(dglm_ctx *c, ws_fn ws, st_fn st0, double *dt_hist, int hist_idx) {
+  const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
+  int tk = timer_start(c, 0);
+  ws(c, c->u[c->cur], c->maxv);
+  timer_stop(c, tk);
+  tk = timer_start(c, 1);
+  hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
+                     c->nblk_ws, c->scal, cfl_fac, c->p.tmax, dt_hist, (int *)nullptr, (int *)nullptr, hist_idx);
+  timer_stop(c, tk);
+  for (int s = 0; s < 5; s++) {
+    StageArgs a = stage_args(c, s);
+    tk = timer_start(c, 2);
+    st0(c, a);
+    timer_stop(c, tk);
+    c->cur ^= 1;
+  }
+}
+
+void drop_graphs(dglm_ctx *c) {
+  for (int g = 0; g < 2; g++)
+    if (c->graph[g]) {
+      hipGraphExecDestroy(c->graph[g]);
+      c->graph[g] = nullptr;
+    }
+}
+
+// Capture two steps (10 stages: the state returns to the same buffer) into
+// an executable graph.  Only kernel launches are recorded.
+int build_graph(dglm_ctx *c, ws_fn ws, st_fn st0) {
+  const int start = c->cur;
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(c, ws, st0, nullptr, 0);
+  enqueue_step(c, ws, st0, nullptr, 0);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->cur = start;
+  if (e != hipSuccess) return fail(DGLM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->graph[start], g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->graph[start] = nullptr;
+    return fail(DGLM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return DGLM_OK;
+}
+}  // namespace
+
 extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
   if (!c || nsteps < 0) return fail(DGLM_ERR_ARG, "dglm_step: bad arguments");
   ws_fn ws;
@@ -749,23 +807,27 @@ extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
     if (rc) return rc;
     c->dt_hist_cap = nsteps;
   }
-  const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
-  for (int st = 0; st < nsteps; st++) {
-    int tk = timer_start(c, 0);
-    ws(c, c->u[c->cur], c->maxv);
-    timer_stop(c, tk);
-    tk = timer_start(c, 1);
-    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
-                       c->nblk_ws, c->scal, cfl_fac, c->p.tmax, dt_hist ? c->dt_hist : nullptr, (int *)nullptr,
-                       (int *)nullptr, st);
-    timer_stop(c, tk);
-    for (int s = 0; s < 5; s++) {
-      StageArgs a = stage_args(c, s);
-      tk = timer_start(c, 2);
-      st0(c, a);
-      timer_stop(c, tk);
-      c->cur ^= 1;
+  int st = 0;
+  // Graph replay when nothing per-step has to come back (no dt history, no
+  // per-kernel timing).  The first step runs eagerly so launch shapes are
+  // fixed before capture.
+  const bool graphs = c->use_graph && !dt_hist && !c->timing;
+  if (graphs && nsteps >= 3) {
+    if (!c->graph[c->cur ^ 1] && !c->graph[c->cur]) {
+      enqueue_step(c, ws, st0, nullptr, 0);
+      st++;
     }
+    while (nsteps - st >= 2) {
+      if (!c->graph[c->cur]) {
+        int rc = build_graph(c, ws, st0);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(c->graph[c->cur], c->stream));
+      st += 2;
+    }
+  }
+  for (; st < nsteps; st++) {
+    enqueue_step(c, ws, st0, dt_hist ? c->dt_hist : nullptr, st);
     if (c->ev_class.size() > 8192) {
       int rc = timer_flush(c);
       if (rc) return rc;
@@ -779,24 +841,10 @@ extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
   return DGLM_OK;
 }
 
-extern "C" int dglm_integrals(dglm_ctx *c, const double *J, const double *M, double *out) {
-  if (!c || !J || !M || !out) return fail(DGLM_ERR_ARG, "dglm_integrals: null argument");
-  const int Np = c->Np, K = c->K;
-  std::vector<double> u((size_t)c->nvar * K * Np), w(Np, 0.0);
-  int rc = dglm_get_state(c, u.data());
-  if (rc) return rc;
-  for (int i = 0; i < Np; i++)
-    for (int j = 0; j < Np; j++) w[i] += M[(size_t)j * Np + i];
-  for (int v = 0; v < c->nvar; v++) {
-    double s = 0.0;
-    for (int e = 0; e < K; e++) {
-      double se = 0.0;
-      const double *ue = &u[((size_t)v * K + e) * Np];
-      for (int i = 0; i < Np; i++) se += w[i] * ue[i];
-      s += J[e] * se;
-    }
-    out[v] = s;
-  }
+extern "C" int dglm_set_graphs(dglm_ctx *c, int on) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_set_graphs: null context");
+  c->use_graph = on != 0;
+  if (!c->use_graph) drop_graphs(c);
   return DGLM_OK;
 }
 
@@ -809,6 +857,7 @@ extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
   if (variant == DGLM_STAGE_SIMPLE && c->p.precision != DGLM_FP64)
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: the LDS-staged kernel is FP64 only");
   c->variant = variant;
+  drop_graphs(c);  // captured launches name the previous kernel
   return DGLM_OK;
 }
 
@@ -837,6 +886,34 @@ extern "C" int dglm_get_timing(dglm_ctx *c, double *ms, long *count, int n) {
   for (int i = 0; i < n && i < DGLM_NTIMERS; i++) {
     ms[i] = c->t_ms[i];
     count[i] = c->t_cnt[i];
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_integrals(dglm_ctx *c, const double *J, const double *M, double *out) {
+  if (!c || !J || !M || !out) return fail(DGLM_ERR_ARG, "dglm_integrals: null argument");
+  const int Np = c->Np, K = c->K;
+  HIP_TRY(hipSetDevice(c->dev));
+  std::vector<double> w(Np, 0.0);
+  for (int i = 0; i < Np; i++)
+    for (int j = 0; j < Np; j++) w[i] += M[(size_t)j * Np + i];
+  const int nblk = (K + 255) / 256;
+  int rc;
+  if (!c->d_J && ((rc = dev_alloc(&c->d_J, (size_t)K, "J")) || (rc = dev_alloc(&c->d_w, (size_t)Np, "weights")) ||
+                  (rc = dev_alloc(&c->d_ipart, (size_t)nblk * c->nvar, "integral partials"))))
+    return rc;
+  HIP_TRY(hipMemcpyAsync(c->d_J, J, (size_t)K * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_w, w.data(), (size_t)Np * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(integrals_kernel, dim3(nblk), dim3(256), 0, c->stream, K, Np, c->nvar, c->u[c->cur], c->d_J,
+                     c->d_w, c->d_ipart);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> part((size_t)nblk * c->nvar);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->d_ipart, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < c->nvar; v++) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(size_t)b * c->nvar + v];
+    out[v] = s;
   }
   return DGLM_OK;
 }
@@ -1017,6 +1094,7 @@ AdjArgs adj_args(dglm_ctx *c) {
 
 extern "C" int dglm_adjoint_rhs(dglm_ctx *c, const double *u, const double *rbar, double *ubar, double *taubar) {
   if (!c || !u || !rbar || !ubar) return fail(DGLM_ERR_ARG, "dglm_adjoint_rhs: null argument");
+  if (c->p.flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: built for the LM flux only");
   HIP_TRY(hipSetDevice(c->dev));
   int rc = ensure_adjoint_buffers(c);
   if (rc) return rc;
@@ -1051,6 +1129,7 @@ extern "C" int dglm_adjoint_rhs(dglm_ctx *c, const double *u, const double *rbar
 
 extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
   if (!c || nsteps <= 0 || every <= 0) return fail(DGLM_ERR_ARG, "dglm_adjoint_record: bad arguments");
+  if (c->p.flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: built for the LM flux only");
   HIP_TRY(hipSetDevice(c->dev));
   int rc = ensure_adjoint_buffers(c);
   if (rc) return rc;

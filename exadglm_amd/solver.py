@@ -31,13 +31,13 @@ class Solver:
     """
 
     def __init__(self, problem=None, physics="euler", N=None, cfl=0.8, gamma=1.4, tau_scale=1.2, tmax=math.inf,
-                 precision="fp64", device=0, arrays=None, K=None, datain_dir=None, matrix_order="c"):
+                 precision="fp64", device=0, arrays=None, K=None, datain_dir=None, matrix_order="c", flux="lm"):
         L = A.lib()
         p = A.Params()
         p.N = N if N is not None else problem.N
         p.physics = _PHYS[physics]
         p.precision = _PREC[precision]
-        p.flux = A.DGLM_FLUX_LM
+        p.flux = {"lm": A.DGLM_FLUX_LM, "rusanov": A.DGLM_FLUX_RUSANOV}[flux]
         p.cfl, p.gamma, p.tau_scale, p.tmax = cfl, gamma, tau_scale, tmax
         p.device = device
         self.params = p
@@ -145,6 +145,9 @@ class Solver:
         cnt = (C.c_long * 3)()
         A.check(A.lib().dglm_get_timing(self._ctx, ms, cnt, 3))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(("wavespeed", "dt", "stage"))}
+
+    def set_graphs(self, on=True):
+        A.check(A.lib().dglm_set_graphs(self._ctx, 1 if on else 0))
 
     def sync(self):
         A.check(A.lib().dglm_sync(self._ctx))

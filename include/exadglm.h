@@ -36,8 +36,12 @@ enum { DGLM_ADVECTION = 0, DGLM_EULER = 1 };
 enum { DGLM_FP64 = 0, DGLM_FP32 = 1, DGLM_BF16X3 = 2 };
 
 /* Numerical flux.  DGLM_FLUX_LM is the reference's Lagrange-multiplier flux
- * (n.F(ub) - tau (ub - u-)) Fscale, dim3/src/cpp/euler_dglm.cpp:110-183. */
-enum { DGLM_FLUX_LM = 0 };
+ * (n.F(ub) - tau (ub - u-)) Fscale, dim3/src/cpp/euler_dglm.cpp:110-183.
+ * DGLM_FLUX_RUSANOV is the textbook local Lax-Friedrichs flux
+ * (n.F(u-) + n.F(u+))/2 + lambda/2 (u- - u+), lambda = max(|v.n| + c) over the
+ * two traces (exact upwinding for advection); it has no reference
+ * counterpart and no adjoint. */
+enum { DGLM_FLUX_LM = 0, DGLM_FLUX_RUSANOV = 1 };
 
 const char *dglm_last_error(void);
 int dglm_version(void);
@@ -156,8 +160,13 @@ int dglm_rhs(dglm_ctx *ctx, const double *u, double *rhs, double *maxv, double *
  * dt_hist [nsteps] may be NULL.  Steps after t reaches tmax are no-ops. */
 int dglm_step(dglm_ctx *ctx, int nsteps, double *dt_hist);
 
-/* Conserved integrals sum_e J_e 1^T M u_e per variable (FP64), out [nvar]. */
+/* Conserved integrals sum_e J_e 1^T M u_e per variable, FP64 on the device
+ * with a fixed-order reduction; J [K], M [Np*Np]; out [nvar]. */
 int dglm_integrals(dglm_ctx *ctx, const double *J, const double *M, double *out);
+
+/* dglm_step replays a captured two-step hipGraph when no dt history or
+ * timing is requested (default on). */
+int dglm_set_graphs(dglm_ctx *ctx, int on);
 
 int dglm_sync(dglm_ctx *ctx);
 

@@ -327,7 +327,7 @@ static void euler_jv(const double U[5], double nx, double ny, double nz, const d
   out[4] = vn * (gm * 0.5 * q2 - H) * d[0] + H * ndm - gm * vn * vdm + G * vn * d[4];
 }
 
-int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, double *drhs) {
+int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, const double *tau, double *drhs) {
   const int K = P->K, Np = P->Np, Nfp = P->Nfp, NFF = NFACE * Nfp, nv = P->nvar;
   const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
   work_t w;
@@ -335,8 +335,12 @@ int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, do
     work_free(&w);
     return -1;
   }
-  calc_maxv(P, u, w.maxv, w.maxv_fs);
-  update_tau(P, w.maxv, w.tau);
+  if (tau) {
+    memcpy(w.tau, tau, (size_t)K * NFACE * sizeof(double));
+  } else {
+    calc_maxv(P, u, w.maxv, w.maxv_fs);
+    update_tau(P, w.maxv, w.tau);
+  }
   /* dB volume -> w.fR/fS/fT, dB surface -> w.fLM */
 #pragma omp parallel for
   for (int ei = 0; ei < K; ei++) {

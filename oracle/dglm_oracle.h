@@ -28,8 +28,10 @@ int oracle_rhs(const oracle_problem *P, const double *u, double *rhs, double *ma
 /* RHS at u with the face penalty tau [K*4] given instead of computed. */
 int oracle_rhs_tau(const oracle_problem *P, const double *u, const double *tau, double *rhs);
 
-/* Tangent-linear RHS with tau frozen at u: drhs = dR/du(u) du. */
-int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, double *drhs);
+/* Tangent-linear RHS with tau frozen: drhs = dR/du(u; tau) du.  tau [K*4] is
+ * the penalty to hold (the one of the step start inside a Runge-Kutta step);
+ * NULL takes the tau that u itself gives. */
+int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, const double *tau, double *drhs);
 
 /* nsteps of runRK on (u, k) in place; t and tstep advance; steps once t has
  * reached tmax are skipped (dt_hist entry 0). */

@@ -168,12 +168,17 @@ class RefSolver:
         return self._call("run"), self.dataout
 
 
-def rhs_tl(oracle, u, du):
-    """Tangent-linear RHS of the oracle (tau frozen at u)."""
+def rhs_tl(oracle, u, du, tau=None):
+    """Tangent-linear RHS of the oracle with tau frozen (at the given [K, 4]
+    penalty, else at the one u gives)."""
     u = np.ascontiguousarray(u, dtype=np.float64).reshape(oracle.nvar, -1)
     du = np.ascontiguousarray(du, dtype=np.float64).reshape(oracle.nvar, -1)
     out = np.empty_like(u)
-    if lib().oracle_rhs_tl(C.byref(oracle.P), _d(u), _d(du), _d(out)) != 0:
+    tp = None
+    if tau is not None:
+        tau = np.ascontiguousarray(tau, dtype=np.float64)
+        tp = _d(tau)
+    if lib().oracle_rhs_tl(C.byref(oracle.P), _d(u), _d(du), tp, _d(out)) != 0:
         raise MemoryError("oracle_rhs_tl allocation failed")
     return out
 

@@ -18,6 +18,17 @@ def smooth_state(pb, physics, seed=0, gamma=1.4):
     return np.stack([rho, rho * vx, rho * vy, rho * vz, E])
 
 
+def noisy_state(pb, physics, seed=0, eps=1e-2):
+    """smooth_state with independent relative noise eps per node: the traces
+    on the two sides of a face differ, so the wave-speed maxima (per face,
+    and the global one that sets dt) are attained at single nodes and faces.
+    On a continuous state the two sides of every face carry the same nodes,
+    the global maximum of lambda*Fscale is a tie between a face and its
+    neighbour, and dt(u) is not differentiable there."""
+    u = smooth_state(pb, physics, seed)
+    return u * (1 + eps * np.random.default_rng(seed + 1000).standard_normal(u.shape))
+
+
 def rel_err(a, b):
     return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 

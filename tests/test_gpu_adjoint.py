@@ -2,13 +2,13 @@
 tangent-linear RHS, trajectory dot products (advection), central finite
 differences of the real forward solver (Euler, including the tau(u) and
 dt(u) paths), checkpoint invariance, and the torch.autograd binding."""
-import math
 
 import numpy as np
 import pytest
 
-from helpers import rel_err, smooth_state
-from exadglm_amd.problem import cube_problem, isentropic_vortex, uniform_problem, vortex_box
+from helpers import noisy_state, rel_err, smooth_state
+from traj_tl import Trajectory
+from exadglm_amd.problem import cube_problem
 from exadglm_amd.solver import Solver
 import pyoracle as O
 
@@ -43,10 +43,12 @@ def test_adjoint_rhs_dot_product(gpu, physics, N):
 
 
 def test_adjoint_tau_sensitivity(gpu):
-    """taubar = (dR/dtau)^T rbar against differences of the oracle RHS in tau."""
+    """taubar = (dR/dtau)^T rbar against differences of the oracle RHS in tau,
+    on a discontinuous state (on a continuous one u- = u+ and the tau term
+    vanishes)."""
     _say("tau adjoint")
     pb = cube_problem(3, 3)
-    u = smooth_state(pb, "euler", seed=61)
+    u = noisy_state(pb, "euler", seed=61)
     rng = np.random.default_rng(7)
     rbar = rng.standard_normal(u.shape)
     dtau = rng.standard_normal((pb.K, 4))
@@ -78,26 +80,29 @@ def test_advection_trajectory_dot_product(gpu):
     assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(Sdu) * np.linalg.norm(w), (lhs, rhs)
 
 
-def _euler_setup(N=3, nh=4):
-    pb = uniform_problem(N, nh + 1, nh + 1, nh + 1, box=vortex_box(nh))
-    a = pb.arrays
-    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
-    return pb, u0
+def _euler_setup(N=3, nh=3):
+    """A small Euler problem on a state without wave-speed ties, where the
+    trajectory is differentiable (tau and dt maxima at single nodes)."""
+    pb = cube_problem(N, nh)
+    return pb, noisy_state(pb, "euler", seed=3)
 
 
 def _objective(u, wts):
     return 0.5 * float(np.sum(wts * u * u))
 
 
-def test_euler_gradient_vs_finite_differences(gpu):
-    """J = 1/2 sum w u(T)^2 after 20 steps: adjoint gradient vs central
-    differences (h = 1e-6 relative) of the actual forward solver along random
-    directions; the exact adjoint follows the tau and dt argmax paths."""
-    _say("Euler gradient vs FD")
+def test_euler_gradient_vs_tangent_linear_and_fd(gpu):
+    """J = 1/2 sum w u(T)^2 after 20 steps.  The adjoint gradient along random
+    directions equals the CPU tangent-linear of the oracle trajectory (exact
+    derivative of the same discrete map, tau(u) and dt(u) paths included) to
+    rounding, and central differences (h = 1e-6 relative) of the GPU forward
+    itself to truncation."""
+    _say("Euler gradient vs tangent-linear and FD")
     pb, u0 = _euler_setup()
     rng = np.random.default_rng(5)
     wts = rng.uniform(0.5, 1.5, u0.shape)
     nsteps = 20
+    T = Trajectory(pb, "euler", cfl=0.4)
     with Solver(pb, "euler", cfl=0.4) as s:
         s.set_state(u0)
         s.adjoint_record(nsteps, 5)
@@ -110,16 +115,16 @@ def test_euler_gradient_vs_finite_differences(gpu):
             s.step(nsteps)
             return _objective(s.get_state(), wts)
 
-        fails = 0
-        for q in range(4):
+        for q in range(3):
             d = rng.standard_normal(u0.shape) * np.abs(u0)
+            uTc, duT = T.run(u0, nsteps, d)
+            tl = dot(wts * uTc, duT)
             h = 1e-6
             fd = (J(u0 + h * d) - J(u0 - h * d)) / (2 * h)
             ad = dot(grad, d)
-            err = abs(fd - ad) / abs(fd)
-            _say(f"  direction {q}: fd={fd:.12e} adjoint={ad:.12e} rel={err:.2e}")
-            fails += err > 1e-6
-    assert fails == 0
+            _say(f"  direction {q}: tl={tl:.15e} adjoint={ad:.15e} fd={fd:.12e}")
+            assert abs(ad - tl) <= 1e-10 * abs(tl), (ad, tl)
+            assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
 
 
 def test_checkpoint_interval_does_not_change_gradient(gpu):
@@ -137,11 +142,14 @@ def test_checkpoint_interval_does_not_change_gradient(gpu):
 
 
 def test_frozen_coefficient_mode(gpu):
-    """Freezing tau and dt changes the Euler gradient (they depend on u) but
-    not the advection one (they do not)."""
+    """With tau and dt frozen the adjoint is the transpose of the CPU
+    tangent-linear with frozen coefficients (dot product to rounding), and it
+    differs from the exact-mode gradient."""
     _say("frozen-coefficient adjoint")
     pb, u0 = _euler_setup()
-    w = np.random.default_rng(2).standard_normal(u0.shape)
+    rng = np.random.default_rng(2)
+    w = rng.standard_normal(u0.shape)
+    d = rng.standard_normal(u0.shape) * np.abs(u0)
     out = []
     for exact in (True, False):
         with Solver(pb, "euler", cfl=0.4) as s:
@@ -150,6 +158,11 @@ def test_frozen_coefficient_mode(gpu):
             s.adjoint_record(6, 3)
             out.append(s.adjoint_backward(w))
     assert rel_err(out[1], out[0]) > 1e-8
+    T = Trajectory(pb, "euler", cfl=0.4)
+    for g, frozen in zip(out, (False, True)):
+        _, duT = T.run(u0, 6, d, frozen=frozen)
+        tl = dot(w, duT)
+        assert abs(dot(g, d) - tl) <= 1e-10 * abs(tl), (frozen, dot(g, d), tl)
 
 
 def test_torch_autograd(gpu):

@@ -174,3 +174,41 @@ def test_oracle_tangent_linear_catches_planted_error():
     tl = O.rhs_tl(O.Oracle(pb, "euler"), u, du)
     tl_bad = O.rhs_tl(O.Oracle(pb, "euler", gamma=1.4 * (1 + 1e-6)), u, du)
     assert rel_err(tl_bad, tl) > 1e-8
+
+
+def test_trajectory_tangent_linear_vs_fd():
+    """The CPU tangent-linear of a whole Euler LSRK trajectory (the tau(u) and
+    dt(u) paths included) against central differences of the oracle
+    forward, on a state without wave-speed ties; with tau and dt frozen it
+    matches differences of the fixed-coefficient forward instead, and then
+    misses the full derivative by far more than the tolerance (so dropping
+    the tau/dt paths would fail the first check)."""
+    from traj_tl import Trajectory
+
+    from helpers import noisy_state
+
+    pb = cube_problem(3, 2)
+    u0 = noisy_state(pb, "euler", seed=3)
+    rng = np.random.default_rng(5)
+    wts = rng.uniform(0.5, 1.5, u0.shape)
+    T = Trajectory(pb, "euler", cfl=0.4)
+    n = 8
+    uT, _ = T.run(u0, n)
+    assert np.array_equal(uT, O.Oracle(pb, "euler", cfl=0.4).run(u0, n)[0])
+    coeffs = list(T.coeffs)
+
+    def J(x, fixed=None):
+        return 0.5 * float(np.sum(wts * T.run(x, n, coeffs=fixed)[0] ** 2))
+
+    h = 1e-6
+    for q in range(2):
+        d = rng.standard_normal(u0.shape) * np.abs(u0)
+        uT, duT = T.run(u0, n, d)
+        tl = float(np.sum(wts * uT * duT))
+        fd = (J(u0 + h * d) - J(u0 - h * d)) / (2 * h)
+        assert abs(tl - fd) <= 1e-6 * abs(fd), (tl, fd)
+        uT, duT = T.run(u0, n, d, frozen=True)
+        tlf = float(np.sum(wts * uT * duT))
+        fdf = (J(u0 + h * d, coeffs) - J(u0 - h * d, coeffs)) / (2 * h)
+        assert abs(tlf - fdf) <= 1e-6 * abs(fdf), (tlf, fdf)
+        assert abs(tlf - fd) > 1e-4 * abs(fd)
