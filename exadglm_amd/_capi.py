@@ -28,6 +28,7 @@ DGLM_FLUX_RUSANOV = 1
 DGLM_STAGE_AUTO = 0
 DGLM_STAGE_SIMPLE = 1
 DGLM_STAGE_MFMA = 2
+DGLM_STAGE_MFMA_WS = 3
 
 _ERRNAMES = {
     DGLM_ERR_ARG: "DGLM_ERR_ARG",

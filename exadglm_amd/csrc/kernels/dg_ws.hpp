@@ -1,0 +1,522 @@
+// Warp-specialised FP64 stage kernel for the Euler equations (gfx950), and
+// the LDS-staged wave-speed kernel.
+//
+// Stage kernel: one persistent workgroup per CU, double-buffered LDS slots of
+// 16 elements ("batches"), two wave roles that meet at one barrier a batch:
+//
+//   producer waves (NPW)  stage batch b+1 into the free slot: node values in
+//       node-major [node][element] order, primitives 1/rho and p, metrics,
+//       tau per face, and the LM surface flux of all five variables at every
+//       face node -- the exterior trace is gathered from global memory once
+//       per face node, not once per variable;
+//   consumer waves (5 x RTF)  wave (v, t) computes row tile t (16 nodes) of
+//       variable v for batch b: the volume-flux B-fragments are formed in
+//       registers from the slot (flux of one node reused by its three
+//       direction k-steps), the surface B-fragments are read from the slot,
+//       v_mfma_f64_16x16x4f64 accumulates against the operator rows held in
+//       registers, and the LSRK update is the epilogue.  The Np - 16*RTF
+//       rows that do not fill a tile are accumulated on the VALU by the t = 0
+//       waves (operator rows in LDS, partial sums reduced across the four
+//       k-lanes at the end) instead of padding a whole MFMA tile.
+//
+// The contraction order is  k = d*Np4 + j  (direction d, node j, Np4 = Np
+// rounded up to 4) followed by the face nodes, so a lane's k-steps of one node
+// in the three directions share one flux evaluation.  Batches are assigned
+// XCD-aware: the CUs of one XCD walk a contiguous range of batches together,
+// so the faces shared between neighbouring batches are served from that
+// XCD's L2.
+//
+// Reference semantics as in dg_fp64.hpp (euler_dglm.cpp:6-302,
+// euler_time.cpp:15-38).
+#pragma once
+#include "dg_mfma.hpp"
+
+namespace dglm {
+
+template <int N>
+struct WsShape {
+  using O = Ord<N>;
+  static constexpr int Np = O::Np, Nfp = O::Nfp, NFF = O::NFF;
+  static constexpr int G = (Np + 3) / 4;  // node groups = k-steps per volume direction
+  static constexpr int Np4 = 4 * G;
+  static constexpr int KV = 3 * Np4;        // volume rows of the contraction
+  static constexpr int KSS = (NFF + 3) / 4;  // surface k-steps
+  static constexpr int NFF4 = 4 * KSS;
+  static constexpr int KS = 3 * G + KSS;
+  static constexpr int KDP = 4 * KS;
+  static constexpr int RTF = Np / 16;         // full MFMA row tiles
+  static constexpr int REM = Np - 16 * RTF;   // rows accumulated on the VALU
+  static constexpr int NC = 5 * RTF;          // consumer waves
+  static constexpr int NPW = 2;               // producer waves
+  static constexpr int NW = NC + NPW;
+  static constexpr int NT = 64 * NW;
+  static constexpr int EB = 16;               // elements per batch (MFMA columns)
+  static constexpr int LS = 17;               // row stride of [node][element] arrays (bank spread)
+  static constexpr int EPW = EB / NPW;        // elements staged per producer wave
+};
+
+template <int N>
+struct WsSlot {
+  using S = WsShape<N>;
+  double u[5][S::Np * S::LS];
+  double inv[S::Np * S::LS];
+  double p[S::Np * S::LS];
+  double fs[5][S::NFF4 * S::EB];  // surface rows, [face node][element]
+  double met[S::EB * 9];
+  double fgeo[S::EB * 16];        // per face: nx, ny, nz, Fscale
+  double tau[S::EB * 4];
+  int conn[S::EB * 4];
+};
+
+template <int N>
+constexpr bool ws_available() {
+  using S = WsShape<N>;
+  return S::RTF >= 1 && S::NT <= 1024 && 2 * sizeof(WsSlot<N>) + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp <= 160 * 1024;
+}
+
+// Host-side: operator rows in the k order of the kernel.
+//   afrag[(t*KS + ks)*64 + l] = A'[16t + l%16][4ks + l/16]   (t < RTF)
+//   arem[r*KDP + k]            = A'[16*RTF + r][k]           (r < REM)
+// with A'[i][d*Np4 + j] = op[(d*Np + j)*Np + i] (j < Np) and
+// A'[i][KV + m] = op[(3Np + m)*Np + i] (m < NFF), zero elsewhere.
+template <int N>
+inline double ws_op(const double *opT, int i, int k) {
+  using S = WsShape<N>;
+  if (k < S::KV) {
+    const int d = k / S::Np4, j = k - d * S::Np4;
+    return j < S::Np ? opT[(size_t)(d * S::Np + j) * S::Np + i] : 0.0;
+  }
+  const int m = k - S::KV;
+  return m < S::NFF ? opT[(size_t)(3 * S::Np + m) * S::Np + i] : 0.0;
+}
+
+template <int N>
+inline void ws_arrange(const double *opT, double *afrag, double *arem) {
+  using S = WsShape<N>;
+  for (int t = 0; t < S::RTF; t++)
+    for (int ks = 0; ks < S::KS; ks++)
+      for (int l = 0; l < 64; l++)
+        afrag[((size_t)t * S::KS + ks) * 64 + l] = ws_op<N>(opT, 16 * t + (l & 15), 4 * ks + (l >> 4));
+  for (int r = 0; r < S::REM; r++)
+    for (int k = 0; k < S::KDP; k++) arem[(size_t)r * S::KDP + k] = ws_op<N>(opT, 16 * S::RTF + r, k);
+}
+
+// Producer: stage batch b into slot sl.  Wave pw owns elements pw, pw+NPW, ...
+// and reads back only LDS entries it wrote itself, so a wave-level fence is
+// the only synchronisation inside the producer.
+template <int N>
+__device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *s_fmx, int b, int pw, int lane) {
+  using S = WsShape<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, LS = S::LS, EB = S::EB, NPW = S::NPW, EPW = S::EPW;
+  const int K = a.K;
+  const size_t KNp = (size_t)K * Np;
+  const int e0 = b * EB;
+  const int ne = min(EB, K - e0);
+
+  for (int idx = lane; idx < EPW * GEO_STRIDE; idx += 64) {
+    const int q = idx / GEO_STRIDE, c = idx - q * GEO_STRIDE, e = pw + NPW * q;
+    const double gv = e < ne ? a.geo[(size_t)(e0 + e) * GEO_STRIDE + c] : 0.0;
+    if (c < 9)
+      sl.met[e * 9 + c] = gv;
+    else
+      sl.fgeo[e * 16 + c - 9] = gv;
+  }
+  for (int idx = lane; idx < EPW * 4; idx += 64) {
+    const int q = idx >> 2, f = idx & 3, e = pw + NPW * q;
+    int cc = 0;
+    double tau = 0.0;
+    if (e < ne) {
+      cc = a.conn[(size_t)(e0 + e) * 4 + f];
+      tau = 0.25 * (a.maxv[(size_t)(e0 + e) * 4 + f] + a.maxv[(size_t)conn_nbr(cc) * 4 + conn_face(cc)]) *
+            a.tau_scale;
+    }
+    sl.conn[e * 4 + f] = cc;
+    sl.tau[e * 4 + f] = tau;
+  }
+
+  // Node values (all loads issued before any use) and primitives.
+  constexpr int NPAIR = EPW * Np;
+  constexpr int NR = (NPAIR + 63) / 64;
+  double uv[NR][5];
+#pragma unroll
+  for (int s = 0; s < NR; s++) {
+    const int P = lane + 64 * s;
+    const int q = P / Np, j = P - q * Np, e = pw + NPW * q;
+    const bool ok = P < NPAIR && e < ne;
+    const size_t gi = (size_t)(e0 + e) * Np + j;
+#pragma unroll
+    for (int v = 0; v < 5; v++) uv[s][v] = ok ? a.u_in[v * KNp + gi] : (v == 0 ? 1.0 : 0.0);
+  }
+#pragma unroll
+  for (int s = 0; s < NR; s++) {
+    const int P = lane + 64 * s;
+    if (P < NPAIR) {
+      const int q = P / Np, j = P - q * Np, e = pw + NPW * q;
+      const int o = j * LS + e;
+      const double r = uv[s][0], mx = uv[s][1], my = uv[s][2], mz = uv[s][3], E = uv[s][4];
+      const double inv = 1.0 / r;
+      sl.u[0][o] = r;
+      sl.u[1][o] = mx;
+      sl.u[2][o] = my;
+      sl.u[3][o] = mz;
+      sl.u[4][o] = E;
+      sl.inv[o] = inv;
+      sl.p[o] = (a.gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  // LM (or Rusanov) surface flux of the five variables at each face node,
+  // times Fscale.  Exterior traces are gathered CH pairs per lane at a time.
+  constexpr int NSP = EPW * S::NFF4;
+  constexpr int NRS = (NSP + 63) / 64;
+  constexpr int CH = 4;
+#pragma unroll 1
+  for (int s0 = 0; s0 < NRS; s0 += CH) {
+    double up[CH][5];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const int P = lane + 64 * (s0 + c);
+      const int q = P / S::NFF4, m = P - q * S::NFF4, e = pw + NPW * q;
+      const bool ok = s0 + c < NRS && P < NSP && e < ne && m < NFF;
+      size_t pidx = 0;
+      if (ok) {
+        const int f = m / Nfp, jf = m - f * Nfp;
+        const int cc = sl.conn[e * 4 + f];
+        pidx = (size_t)conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
+      }
+#pragma unroll
+      for (int v = 0; v < 5; v++) up[c][v] = ok ? a.u_in[v * KNp + pidx] : (v == 0 ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const int P = lane + 64 * (s0 + c);
+      if (s0 + c < NRS && P < NSP) {
+        const int q = P / S::NFF4, m = P - q * S::NFF4, e = pw + NPW * q;
+        double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (e < ne && m < NFF) {
+          const int f = m / Nfp, jf = m - f * Nfp;
+          const int o = s_fmx[(f * 4 + 3) * Nfp + jf] * LS + e;
+          const double *g = &sl.fgeo[e * 16 + f * 4];
+          double uM[5];
+#pragma unroll
+          for (int v = 0; v < 5; v++) uM[v] = sl.u[v][o];
+          if (a.flux == 1) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) fl[v] = euler_rusanov(v, uM, up[c], g, a.gamma) * g[3];
+          } else {
+            const double tau = sl.tau[e * 4 + f];
+            double ub[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + up[c][v]);
+            const double inv = 1.0 / ub[0];
+            const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
+            const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
+            double nF[5];
+            nF[0] = g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3];
+            nF[1] = ub[1] * vn + g[0] * p;
+            nF[2] = ub[2] * vn + g[1] * p;
+            nF[3] = ub[3] * vn + g[2] * p;
+            nF[4] = (ub[4] + p) * vn;
+#pragma unroll
+            for (int v = 0; v < 5; v++) fl[v] = (nF[v] - tau * (ub[v] - uM[v])) * g[3];
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) sl.fs[v][m * EB + e] = fl[v];
+      }
+    }
+  }
+}
+
+// Consumer wave (v, t): batches b, b+gsz, ... < hi from alternating slots.
+// WREM: this wave also accumulates the REM rows beyond the full tiles (kept
+// as a separate instantiation so both loops stay straight-line code).
+template <int N, int MODE, bool WREM>
+__device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const double *s_arem,
+                                  const double *__restrict__ afrag, double dt, int b, int hi, int gsz, int v, int t,
+                                  int lane) {
+  using S = WsShape<N>;
+  using P = PrecTraits<PREC_F64>;
+  constexpr int Np = S::Np, LS = S::LS, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV;
+  constexpr int KS = S::KS, KSS = S::KSS, KDP = S::KDP, RTF = S::RTF;
+  constexpr int REM = WREM ? S::REM : 0;
+  constexpr int RA = REM > 0 ? REM : 1;
+  constexpr int RR = (REM + 3) / 4;  // remainder rows per lane
+  const int K = a.K;
+  const size_t KNp = (size_t)K * Np;
+  const int kk = lane >> 4, e = lane & 15;
+
+  double afr[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) afr[ks] = afrag[((size_t)t * KS + ks) * 64 + lane];
+
+  for (int it = 0; b < hi; b += gsz, it++) {
+    const WsSlot<N> &s = sl[it & 1];
+    const int e0 = b * EB;
+    const int ne = min(EB, K - e0);
+    const bool live = e < ne;
+    const size_t gbase = v * KNp + (size_t)(e0 + (live ? e : 0)) * Np;
+
+    double kp[4], kpr[RR > 0 ? RR : 1];
+    if (MODE == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) kp[r] = live ? a.k[gbase + 16 * t + 4 * r + kk] : 0.0;
+#pragma unroll
+      for (int x = 0; x < RR; x++) {
+        const int rr = kk + 4 * x;
+        kpr[x] = (live && rr < REM) ? a.k[gbase + 16 * RTF + rr] : 0.0;
+      }
+    }
+    double g9[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) g9[c] = s.met[e * 9 + c];
+    // The remainder-row operator values are the same every batch; an opaque
+    // zero offset keeps them as LDS reads inside the loop instead of
+    // REM*KS registers held across it.
+    int zoff = 0;
+    asm volatile("" : "+v"(zoff));
+    const double *arw = s_arem + zoff + kk;
+
+    typename P::Acc acc = {0, 0, 0, 0};
+    double racc[RA];
+#pragma unroll
+    for (int r = 0; r < RA; r++) racc[r] = 0.0;
+
+    // ---- volume rows: one flux evaluation per node, three k-steps ----
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const int j = 4 * g + kk;
+      double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+      if (j < Np) {
+        const int o = j * LS + e;
+        const double mx = s.u[1][o], my = s.u[2][o], mz = s.u[3][o];
+        double F, Gf, H;
+        if (v == 0) {
+          F = mx;
+          Gf = my;
+          H = mz;
+        } else {
+          const double inv = s.inv[o], p = s.p[o];
+          const double vx = mx * inv, vy = my * inv, vz = mz * inv;
+          const double c = v == 4 ? s.u[4][o] + p : (v == 1 ? mx : v == 2 ? my : mz);
+          F = c * vx + (v == 1 ? p : 0.0);
+          Gf = c * vy + (v == 2 ? p : 0.0);
+          H = c * vz + (v == 3 ? p : 0.0);
+        }
+        f0 = g9[0] * F + g9[1] * Gf + g9[2] * H;
+        f1 = g9[3] * F + g9[4] * Gf + g9[5] * H;
+        f2 = g9[6] * F + g9[7] * Gf + g9[8] * H;
+      }
+      acc = P::mfma(afr[g], f0, acc);
+      acc = P::mfma(afr[G + g], f1, acc);
+      acc = P::mfma(afr[2 * G + g], f2, acc);
+#pragma unroll
+      for (int r = 0; r < REM; r++) {
+        const double *ar = &arw[r * KDP + 4 * g];
+        racc[r] += ar[0] * f0 + ar[Np4] * f1 + ar[2 * Np4] * f2;
+        // materialise the partial sum here: left alone the compiler sinks
+        // these FMAs to the epilogue and holds every operand until then
+        asm volatile("" : "+v"(racc[r]));
+      }
+      // keep each node group's LDS reads next to its MFMAs (bounds the
+      // registers the scheduler may spend on reads hoisted from later groups)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // ---- surface rows ----
+#pragma unroll
+    for (int ks = 0; ks < KSS; ks++) {
+      const double bf = s.fs[v][(4 * ks + kk) * EB + e];
+      acc = P::mfma(afr[3 * G + ks], bf, acc);
+#pragma unroll
+      for (int r = 0; r < REM; r++) {
+        racc[r] += arw[r * KDP + KV + 4 * ks] * bf;
+        asm volatile("" : "+v"(racc[r]));
+      }
+      if (ks % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // ---- LSRK epilogue (element e, rows 16t + 4r + kk) ----
+    if (live) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = 16 * t + 4 * r + kk;
+        const size_t gi = gbase + i;
+        const double rhs = acc[r];
+        if (MODE == 1) {
+          a.rhs[gi] = rhs;
+        } else {
+          const double kn = a.rka * kp[r] + dt * rhs;
+          a.k[gi] = kn;
+          a.u_out[gi] = s.u[v][i * LS + e] + a.rkb * kn;
+        }
+      }
+    }
+    if (REM > 0) {
+      // partial sums of the four k-lanes of each element
+#pragma unroll
+      for (int r = 0; r < REM; r++) {
+        racc[r] += __shfl_xor(racc[r], 16, 64);
+        racc[r] += __shfl_xor(racc[r], 32, 64);
+      }
+#pragma unroll
+      for (int x = 0; x < RR; x++) {
+        const int rr = kk + 4 * x;
+        double rhs = 0.0;
+#pragma unroll
+        for (int r = 0; r < RA; r++)
+          if (r == rr) rhs = racc[r];
+        if (live && rr < REM) {
+          const int i = 16 * RTF + rr;
+          const size_t gi = gbase + i;
+          if (MODE == 1) {
+            a.rhs[gi] = rhs;
+          } else {
+            const double kn = a.rka * kpr[x] + dt * rhs;
+            a.k[gi] = kn;
+            a.u_out[gi] = s.u[v][i * LS + e] + a.rkb * kn;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+//   MODE 0: LSRK update; MODE 1: write the RHS only.
+// ngrp: number of XCD groups the grid is split into (8, or 1 for plain
+// striding); gridDim.x must be a multiple of ngrp.
+template <int N, int MODE>
+__global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, const double *__restrict__ afrag,
+                                                                   const double *__restrict__ arem, int nbatch,
+                                                                   int ngrp) {
+  using S = WsShape<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp, KDP = S::KDP, RTF = S::RTF, REM = S::REM, NC = S::NC, NT = S::NT;
+
+  __shared__ WsSlot<N> sl[2];
+  __shared__ double s_arem[REM > 0 ? REM * KDP : 1];
+  __shared__ int s_fmx[16 * Nfp];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform: role branches are scalar
+  const size_t KNp = (size_t)a.K * Np;
+
+  if (MODE == 0 && !a.scal->active) {
+    // Past tmax: the stage is the identity on u (k untouched).
+    for (size_t i = (size_t)blockIdx.x * NT + tid; i < 5 * KNp; i += (size_t)gridDim.x * NT) a.u_out[i] = a.u_in[i];
+    return;
+  }
+  const double dt = MODE == 0 ? a.scal->dt : 0.0;
+
+  for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
+  for (int i = tid; i < REM * KDP; i += NT) s_arem[i] = arem[i];
+
+  // Batch schedule: group x = blockIdx % ngrp walks batches [lo, hi) with
+  // the gridDim/ngrp workgroups of the group striding together.
+  const int grp = blockIdx.x % ngrp, gsl = blockIdx.x / ngrp, gsz = gridDim.x / ngrp;
+  const int lo = (int)((long)nbatch * grp / ngrp), hi = (int)((long)nbatch * (grp + 1) / ngrp);
+  int b = lo + gsl;
+
+  __syncthreads();  // s_fmx
+  if (w >= NC && b < hi) ws_produce<N>(a, sl[0], s_fmx, b, w - NC, lane);
+  __syncthreads();
+
+  if (w >= NC) {
+    // producer waves: stage the batch after the one being consumed
+    for (int it = 0; b < hi; b += gsz, it++) {
+      if (b + gsz < hi) ws_produce<N>(a, sl[(it + 1) & 1], s_fmx, b + gsz, w - NC, lane);
+      __syncthreads();
+    }
+    return;
+  }
+  const int v = w / RTF, t = w - v * RTF;
+  if (REM > 0 && t == 0)
+    ws_consume<N, MODE, true>(a, sl, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
+  else
+    ws_consume<N, MODE, false>(a, sl, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
+}
+
+// --------------------------------------------------------------------------
+// LDS-staged wave speed: a block converts the node values of WE elements to
+// |v|^2 and |c|^2 with coalesced loads, then one thread per face takes the
+// maxima over the face nodes.  Same arithmetic and tie rule as
+// wavespeed_kernel.
+// --------------------------------------------------------------------------
+__host__ __device__ constexpr int wavespeed_elems(int Np) { return Np <= 35 ? 64 : 32; }
+
+template <int N, int PHYS>
+__global__ __launch_bounds__(256) void wavespeed_lds_kernel(int K, const double *__restrict__ u,
+                                                            const double *__restrict__ vel,
+                                                            const double *__restrict__ geo,
+                                                            const int *__restrict__ fmx, double *__restrict__ maxv,
+                                                            double *__restrict__ partial,
+                                                            int *__restrict__ partial_idx, double gamma) {
+  using O = Ord<N>;
+  constexpr int Np = O::Np, Nfp = O::Nfp, WE = wavespeed_elems(O::Np);
+  __shared__ double s_v2[WE * Np];
+  __shared__ double s_c2[PHYS == PHYS_EULER ? WE * Np : 1];
+  __shared__ int s_fm[4 * Nfp];
+  __shared__ double red[256];
+  __shared__ int red_i[256];
+  const int tid = threadIdx.x;
+  const size_t KNp = (size_t)K * Np;
+  const int e0 = blockIdx.x * WE;
+  const int ne = min(WE, K - e0);
+  for (int i = tid; i < 4 * Nfp; i += 256) s_fm[i] = fmx[((i / Nfp) * 4 + 3) * Nfp + i % Nfp];
+  for (int i = tid; i < ne * Np; i += 256) {
+    const size_t idx = (size_t)e0 * Np + i;
+    if (PHYS == PHYS_EULER) {
+      const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx],
+                   u5 = u[4 * KNp + idx];
+      const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
+      const double v2 = vx * vx + vy * vy + vz * vz;
+      const double pres = (gamma - 1.0) * (u5 - 0.5 * u1 * v2);
+      s_v2[i] = v2;
+      s_c2[i] = fabs(gamma * pres / u1);
+    } else {
+      const double vx = vel[idx], vy = vel[KNp + idx], vz = vel[2 * KNp + idx];
+      s_v2[i] = vx * vx + vy * vy + vz * vz;
+    }
+  }
+  __syncthreads();
+  const int fid = e0 * 4 + tid;
+  double lfs = 0.0;
+  if (tid < ne * 4) {
+    const int e = tid >> 2, f = tid & 3;
+    double mv2 = 0.0, mc2 = 0.0;
+    for (int j = 0; j < Nfp; j++) {
+      const int q = e * Np + s_fm[f * Nfp + j];
+      const double v2 = s_v2[q];
+      mv2 = v2 > mv2 ? v2 : mv2;
+      if (PHYS == PHYS_EULER) {
+        const double c2 = s_c2[q];
+        mc2 = c2 > mc2 ? c2 : mc2;
+      }
+    }
+    const double lam = PHYS == PHYS_EULER ? sqrt(mv2) + sqrt(mc2) : sqrt(mv2);
+    maxv[fid] = lam;
+    lfs = lam * geo[(size_t)(e0 + e) * GEO_STRIDE + 9 + f * 4 + 3];
+  }
+  red[tid] = lfs;
+  red_i[tid] = tid < ne * 4 ? fid : 0x7fffffff;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double o = red[tid + s];
+      if (o > red[tid] || (o == red[tid] && red_i[tid + s] < red_i[tid])) {
+        red[tid] = o;
+        red_i[tid] = red_i[tid + s];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    partial[blockIdx.x] = red[0];
+    partial_idx[blockIdx.x] = red_i[0];
+  }
+}
+
+}  // namespace dglm

@@ -21,6 +21,7 @@
 #include "host/refelem.h"
 #include "kernels/dg_fp64.hpp"
 #include "kernels/dg_mfma.hpp"
+#include "kernels/dg_ws.hpp"
 #include "kernels/dg_adjoint.hpp"
 
 #define DGLM_NTIMERS 3  // wave speed, dt finalize, stage
@@ -68,6 +69,9 @@ struct dglm_ctx {
   void *afrag = nullptr;          // operator A-fragments for the MFMA kernel (FP64 or FP32)
   double *rowsum = nullptr;       // FP64 block row sums of the operator (mixed precision)
   int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
+  double *afrag_ws = nullptr;     // operator rows in the warp-specialised kernel's k order
+  double *arem_ws = nullptr;      // rows left to the VALU by that kernel
+  int ws_grid = 0, ws_groups = 1;  // its persistent grid and XCD groups
   struct dglm_adj_tape *tape = nullptr;  // adjoint buffers and recorded trajectory
   bool use_graph = true;                 // replay two-step hipGraphs in dglm_step
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // keyed by the state buffer the step starts from
@@ -223,18 +227,58 @@ namespace {
 
 template <int N, int PHYS>
 void launch_wavespeed(dglm_ctx *c, const double *u, double *maxv) {
-  hipLaunchKernelGGL((wavespeed_kernel<N, PHYS>), dim3(c->nblk_ws), dim3(256), 0, c->stream, c->K, u, c->vel,
+  hipLaunchKernelGGL((wavespeed_lds_kernel<N, PHYS>), dim3(c->nblk_ws), dim3(256), 0, c->stream, c->K, u, c->vel,
                      c->geo, c->fmx, maxv, c->partial, c->partial_idx, c->p.gamma);
 }
 
 // True when the MFMA stage kernel is built for this order.
 inline bool mfma_available(int N) { return N >= 1 && N <= 6; }
 
-// The mixed-precision contraction exists only as an MFMA kernel; FP64 can use
-// either kernel.
-bool use_mfma(const dglm_ctx *c) {
-  if (!mfma_available(c->p.N) || !c->afrag) return false;
-  return c->p.precision != DGLM_FP64 || c->variant != DGLM_STAGE_SIMPLE;
+// True when the warp-specialised kernel exists for this order.
+inline bool ws_built(int N) {
+  switch (N) {
+    case 3:
+      return ws_available<3>();
+    case 4:
+      return ws_available<4>();
+    default:
+      return false;
+  }
+}
+
+// The stage kernel that runs: the request where it exists, else the best
+// available (the mixed-precision contraction exists only as an MFMA kernel).
+int stage_kind(const dglm_ctx *c) {
+  const bool ws = c->afrag_ws != nullptr, mf = mfma_available(c->p.N) && c->afrag != nullptr;
+  if (c->variant == DGLM_STAGE_SIMPLE && c->p.precision == DGLM_FP64) return DGLM_STAGE_SIMPLE;
+  if (c->variant == DGLM_STAGE_MFMA && mf) return DGLM_STAGE_MFMA;
+  if (ws) return DGLM_STAGE_MFMA_WS;
+  return mf ? DGLM_STAGE_MFMA : DGLM_STAGE_SIMPLE;
+}
+
+bool use_mfma(const dglm_ctx *c) { return stage_kind(c) == DGLM_STAGE_MFMA; }
+
+template <int N, int MODE>
+void launch_ws(dglm_ctx *c, const StageArgs &a) {
+  if constexpr (ws_available<N>()) {
+    using S = WsShape<N>;
+    const int nbatch = (c->K + S::EB - 1) / S::EB;
+    if (c->ws_grid == 0) {
+      int dev = 0, cus = 0, occ = 0;
+      hipGetDevice(&dev);
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_ws_kernel<N, 0>, S::NT, 0);
+      c->ws_grid = (cus > 0 ? cus : 256) * (occ > 0 ? occ : 1);
+      c->ws_groups = c->ws_grid % 8 == 0 ? 8 : 1;
+    }
+    int grid = c->ws_grid, groups = c->ws_groups;
+    if (nbatch < grid) {
+      grid = nbatch;
+      groups = 1;
+    }
+    hipLaunchKernelGGL((stage_ws_kernel<N, MODE>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->afrag_ws,
+                       c->arem_ws, nbatch, groups);
+  }
 }
 
 template <int N, int PHYS, int MODE, int PREC>
@@ -255,7 +299,12 @@ void launch_mfma(dglm_ctx *c, const StageArgs &a) {
 
 template <int N, int PHYS, int MODE>
 void launch_stage(dglm_ctx *c, const StageArgs &a) {
-  if (use_mfma(c)) {
+  const int kind = stage_kind(c);
+  if (PHYS == PHYS_EULER && kind == DGLM_STAGE_MFMA_WS) {
+    launch_ws<N, MODE>(c, a);
+    return;
+  }
+  if (kind == DGLM_STAGE_MFMA) {
     if (c->p.precision == DGLM_FP32)
       launch_mfma<N, PHYS, MODE, PREC_F32>(c, a);
     else
@@ -331,7 +380,7 @@ void free_ctx(dglm_ctx *c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
-                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx};
+                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx, c->afrag_ws, c->arem_ws};
   for (void *q : ptrs)
     if (q) hipFree(q);
   free_tape(c->tape);
@@ -490,6 +539,19 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   }
   const size_t afrag_bytes = afr.size() * sizeof(double) + afr32.size() * sizeof(float);
   const void *afrag_src = afr.empty() ? (const void *)afr32.data() : (const void *)afr.data();
+  // Operator rows for the warp-specialised FP64 Euler kernel.
+  std::vector<double> afr_ws, arem_ws;
+  if (p->physics == DGLM_EULER && p->precision == DGLM_FP64 && ws_built(p->N)) {
+    if (p->N == 3) {
+      afr_ws.resize((size_t)WsShape<3>::RTF * WsShape<3>::KS * 64);
+      arem_ws.resize((size_t)WsShape<3>::REM * WsShape<3>::KDP);
+      ws_arrange<3>(op.data(), afr_ws.data(), arem_ws.data());
+    } else {
+      afr_ws.resize((size_t)WsShape<4>::RTF * WsShape<4>::KS * 64);
+      arem_ws.resize((size_t)WsShape<4>::REM * WsShape<4>::KDP);
+      ws_arrange<4>(op.data(), afr_ws.data(), arem_ws.data());
+    }
+  }
 
   dglm_ctx *c = new dglm_ctx();
   c->p = *p;
@@ -500,7 +562,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   c->KD = KD;
   c->nvar = p->physics == DGLM_EULER ? 5 : 1;
   c->dev = p->device;
-  c->nblk_ws = (K * 4 + 255) / 256;
+  c->nblk_ws = (K + wavespeed_elems(Np) - 1) / wavespeed_elems(Np);
   c->h_conn = conn;
   int rc;
   hipError_t he = hipSetDevice(c->dev);
@@ -517,6 +579,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (rc = dev_alloc(&c->conn, conn.size(), "conn")) || (rc = dev_alloc(&c->fmx, fmx.size(), "fmx")) ||
       (rc = dev_alloc(&c->scal, 1, "scalars")) ||
       (afrag_bytes && (rc = dev_alloc((char **)&c->afrag, afrag_bytes, "A-fragments"))) ||
+      (!afr_ws.empty() && (rc = dev_alloc(&c->afrag_ws, afr_ws.size(), "A-fragments (warp-specialised)"))) ||
+      (!arem_ws.empty() && (rc = dev_alloc(&c->arem_ws, arem_ws.size(), "remainder rows"))) ||
       (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "row sums")))) {
     free_ctx(c);
     return rc;
@@ -538,6 +602,10 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       hipMemcpy(c->conn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice),
       hipMemcpy(c->fmx, fmx.data(), fmx.size() * sizeof(int), hipMemcpyHostToDevice),
       c->afrag ? hipMemcpy(c->afrag, afrag_src, afrag_bytes, hipMemcpyHostToDevice) : hipSuccess,
+      c->afrag_ws ? hipMemcpy(c->afrag_ws, afr_ws.data(), afr_ws.size() * sizeof(double), hipMemcpyHostToDevice)
+                  : hipSuccess,
+      c->arem_ws ? hipMemcpy(c->arem_ws, arem_ws.data(), arem_ws.size() * sizeof(double), hipMemcpyHostToDevice)
+                 : hipSuccess,
       c->rowsum ? hipMemcpy(c->rowsum, rsum.data(), rsum.size() * sizeof(double), hipMemcpyHostToDevice)
                 : hipSuccess,
       hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
@@ -850,8 +918,12 @@ extern "C" int dglm_set_graphs(dglm_ctx *c, int on) {
 
 extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
   if (!c) return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: null context");
-  if (variant != DGLM_STAGE_AUTO && variant != DGLM_STAGE_SIMPLE && variant != DGLM_STAGE_MFMA)
+  if (variant != DGLM_STAGE_AUTO && variant != DGLM_STAGE_SIMPLE && variant != DGLM_STAGE_MFMA &&
+      variant != DGLM_STAGE_MFMA_WS)
     return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: unknown variant");
+  if (variant == DGLM_STAGE_MFMA_WS && !c->afrag_ws)
+    return fail(DGLM_ERR_UNSUPPORTED,
+                "dglm_set_stage_kernel: the warp-specialised kernel is FP64 Euler with N = 3 or 4");
   if (variant == DGLM_STAGE_MFMA && !mfma_available(c->p.N))
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: no MFMA kernel for this order");
   if (variant == DGLM_STAGE_SIMPLE && c->p.precision != DGLM_FP64)
@@ -863,7 +935,7 @@ extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
 
 extern "C" int dglm_get_stage_kernel(dglm_ctx *c, int *variant) {
   if (!c || !variant) return fail(DGLM_ERR_ARG, "dglm_get_stage_kernel: null argument");
-  *variant = use_mfma(c) ? DGLM_STAGE_MFMA : DGLM_STAGE_SIMPLE;
+  *variant = stage_kind(c);
   return DGLM_OK;
 }
 

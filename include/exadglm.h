@@ -178,10 +178,13 @@ int dglm_sync(dglm_ctx *ctx);
 int dglm_set_timing(dglm_ctx *ctx, int on);
 
 /* Stage-kernel selection (the reference picks its GPU variant by file prefix,
- * *_nopad_ / *_pad_, dglm_3d_launcher.py:176-205).  AUTO uses the FP64 MFMA
- * kernel where it is built (N <= 5) and the LDS-staged kernel otherwise;
- * dglm_get_stage_kernel reports the kernel that runs. */
-enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2 };
+ * *_nopad_ / *_pad_, dglm_3d_launcher.py:176-205).  SIMPLE: LDS-staged FP64;
+ * MFMA: one workgroup per 16-element batch, operator row tiles in registers
+ * (all orders, FP64 and mixed precision); MFMA_WS: persistent warp-specialised
+ * FP64 Euler kernel (producer waves stage the next batch while consumer waves
+ * run the MFMA contraction; N = 3, 4).  AUTO takes MFMA_WS where it exists,
+ * else MFMA.  dglm_get_stage_kernel reports the kernel that runs. */
+enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2, DGLM_STAGE_MFMA_WS = 3 };
 int dglm_set_stage_kernel(dglm_ctx *ctx, int variant);
 int dglm_get_stage_kernel(dglm_ctx *ctx, int *variant);
 int dglm_get_timing(dglm_ctx *ctx, double *ms, long *count, int n);

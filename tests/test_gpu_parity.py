@@ -12,7 +12,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from helpers import rel_err, rel_err_per_var, smooth_state
+from helpers import noisy_state, rel_err, rel_err_per_var, smooth_state
 from exadglm_amd.problem import (cube_problem, ic_advection_sine, ic_euler_density_wave, isentropic_vortex, l2_error,
                                  uniform_problem, vortex_box)
 from exadglm_amd.solver import Solver
@@ -35,13 +35,16 @@ def euler_errors(u, uo):
     return errs
 
 
-def _variants(N):
-    return ["simple", "mfma"]
+def _variants(N, physics):
+    """Stage kernels built for this order and physics."""
+    return ["simple", "mfma"] + (["mfma_ws"] if physics == "euler" and N in (3, 4) else [])
 
 
 @pytest.mark.parametrize("physics", PHYS)
-@pytest.mark.parametrize("N,variant", [(n, v) for n in range(1, 7) for v in _variants(n)])
+@pytest.mark.parametrize("N,variant", [(n, v) for n in range(1, 7) for v in ("simple", "mfma", "mfma_ws")])
 def test_rhs_parity(gpu, physics, N, variant):
+    if variant not in _variants(N, physics):
+        pytest.skip(f"no {variant} kernel for {physics} N={N}")
     _say(f"rhs parity {physics} N={N} {variant}")
     pb = cube_problem(N, 3)
     u = smooth_state(pb, physics, seed=N + 11)
@@ -224,13 +227,13 @@ def test_timing_and_noop_after_tmax(gpu):
 @pytest.mark.parametrize("physics", PHYS)
 @pytest.mark.parametrize("N", [2, 3, 4, 5])
 def test_steps_both_stage_kernels(gpu, physics, N):
-    """The LDS-staged and the MFMA stage kernels both track the oracle over
-    20 steps (K = 162, a partial last batch)."""
+    """Every stage kernel built for the order tracks the oracle over 20 steps
+    (K = 162, a partial last batch)."""
     _say(f"both kernels {physics} N={N}")
     pb = cube_problem(N, 3)
     u0 = smooth_state(pb, physics, seed=21 + N)
     out = {}
-    for variant in ("simple", "mfma"):
+    for variant in _variants(N, physics):
         with Solver(pb, physics, cfl=0.5) as s:
             s.set_stage_kernel(variant)
             s.set_state(u0)
@@ -306,3 +309,28 @@ def test_mixed_precision_vortex(gpu, N):
     uo, _, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 40)
     errs = euler_errors(u, uo)
     assert max(errs) < 5e-8, errs
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_warp_specialised_kernel_many_batches(gpu, N):
+    """The persistent warp-specialised kernel with more batches than
+    workgroups (XCD-grouped schedule, 500 batches, last one holding 2
+    elements) on a discontinuous state (tau term active): RHS and 3 steps
+    against the oracle."""
+    _say(f"warp-specialised many batches N={N}")
+    pb = cube_problem(N, 11)
+    assert (pb.K + 15) // 16 == 500 and pb.K % 16 == 2
+    u0 = noisy_state(pb, "euler", seed=40 + N)
+    o = O.Oracle(pb, "euler", cfl=0.4)
+    with Solver(pb, "euler", cfl=0.4) as s:
+        s.set_stage_kernel("mfma_ws")
+        r, _, _ = s.rhs(u0)
+        s.set_state(u0)
+        s.step(3)
+        u = s.get_state()
+    ro, _, _ = o.rhs(u0)
+    for e in rel_err_per_var(r, ro):
+        assert e < 1e-12
+    uo, _, _ = o.run(u0, 3)
+    for e in rel_err_per_var(u, uo):
+        assert e < 1e-12
