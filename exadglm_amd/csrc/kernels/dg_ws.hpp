@@ -4,11 +4,17 @@
 // Stage kernel: one persistent workgroup per CU, double-buffered LDS slots of
 // 16 elements ("batches"), two wave roles that meet at one barrier a batch:
 //
-//   producer waves (NPW)  stage batch b+1 into the free slot: node values in
-//       node-major [node][element] order, primitives 1/rho and p, metrics,
-//       tau per face, and the LM surface flux of all five variables at every
-//       face node -- the exterior trace is gathered from global memory once
-//       per face node, not once per variable;
+//   producer waves (NPW = 2, one per half batch of 8 elements)  stage batch
+//       b+1 into the free slot while the consumers work on batch b.  Every
+//       global access of a batch is issued at once -- node values and the
+//       geometry records go straight from global memory into LDS
+//       (global_load_lds_dwordx4, no registers), the exterior traces of all
+//       face nodes and the wave speeds for tau are gathered into registers
+//       with the connectivity prefetched one batch earlier -- so a batch
+//       costs one memory round trip.  Then the primitives 1/rho and p, tau
+//       per face and the LM surface flux of all five variables at every face
+//       node (the exterior trace gathered once per face node, not once per
+//       variable) are written to the slot;
 //   consumer waves (5 x RTF)  wave (v, t) computes row tile t (16 nodes) of
 //       variable v for batch b: the volume-flux B-fragments are formed in
 //       registers from the slot (flux of one node reused by its three
@@ -51,27 +57,29 @@ struct WsShape {
   static constexpr int NW = NC + NPW;
   static constexpr int NT = 64 * NW;
   static constexpr int EB = 16;               // elements per batch (MFMA columns)
-  static constexpr int LS = 17;               // row stride of [node][element] arrays (bank spread)
-  static constexpr int EPW = EB / NPW;        // elements staged per producer wave
+  static constexpr int EPW = EB / NPW;        // elements staged per producer wave (a contiguous half)
 };
 
+// One batch in LDS.  Node arrays are element-major [element][node] (the
+// global order, so a half batch is one contiguous DMA copy per variable); a
+// B-fragment read of 16 elements then strides Np doubles, which spreads the
+// 16 lanes over distinct banks for odd Np.
 template <int N>
 struct WsSlot {
   using S = WsShape<N>;
-  double u[5][S::Np * S::LS];
-  double inv[S::Np * S::LS];
-  double p[S::Np * S::LS];
+  alignas(16) double u[5][S::EB * S::Np];
+  alignas(16) double geo[S::EB * GEO_STRIDE];  // 9 metrics, 4 x (nx, ny, nz, Fscale)
+  double inv[S::EB * S::Np];
+  double p[S::EB * S::Np];
   double fs[5][S::NFF4 * S::EB];  // surface rows, [face node][element]
-  double met[S::EB * 9];
-  double fgeo[S::EB * 16];        // per face: nx, ny, nz, Fscale
   double tau[S::EB * 4];
-  int conn[S::EB * 4];
 };
 
 template <int N>
 constexpr bool ws_available() {
   using S = WsShape<N>;
-  return S::RTF >= 1 && S::NT <= 1024 && 2 * sizeof(WsSlot<N>) + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp <= 160 * 1024;
+  return S::RTF >= 1 && S::NT <= 1024 &&
+         2 * sizeof(WsSlot<N>) + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp + 4 * 2 * S::EB * 4 <= 160 * 1024;
 }
 
 // Host-side: operator rows in the k order of the kernel.
@@ -101,133 +109,157 @@ inline void ws_arrange(const double *opT, double *afrag, double *arem) {
     for (int k = 0; k < S::KDP; k++) arem[(size_t)r * S::KDP + k] = ws_op<N>(opT, 16 * S::RTF + r, k);
 }
 
-// Producer: stage batch b into slot sl.  Wave pw owns elements pw, pw+NPW, ...
-// and reads back only LDS entries it wrote itself, so a wave-level fence is
-// the only synchronisation inside the producer.
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Copy `bytes` (a multiple of 8, at most 64*16*NI) from global src to LDS dst
+// with one global_load_lds_dwordx4 per KiB: lane l of instruction q moves the
+// 16 bytes at offset (64q + l)*16; lanes past the end are masked off (a final
+// 8-byte tail is moved by a plain load and store).  src, dst 16-byte aligned;
+// dst is wave-uniform.
+template <int NI>
+__device__ inline void dma_copy(const double *src, double *dst, int bytes, int lane) {
+  if (((uintptr_t)src & 15) != 0) {
+    // source not 16-byte aligned (odd element counts): plain copy
+    for (int i = lane; i < (bytes >> 3); i += 64) dst[i] = src[i];
+    return;
+  }
+  const int n16 = bytes >> 4;
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    const int i = 64 * q + lane;
+    if (i < n16)
+      __builtin_amdgcn_global_load_lds((const void *)(src + 2 * i), (lds_void *)(dst + 128 * q), 16, 0, 0);
+  }
+  if ((bytes & 15) && lane == 0) dst[2 * n16] = src[2 * n16];
+}
+
+// Producer wave pw stages elements [8pw, 8pw+8) of batch b into slot sl.
+// cb_cur holds the batch's packed connectivity (loaded by the previous call);
+// the connectivity of batch nb (the next one this wave stages, < 0: none) is
+// loaded into cb_next.  Every LDS entry the wave reads back it wrote itself,
+// so a wave-level fence is the only synchronisation inside the producer.
 template <int N>
-__device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *s_fmx, int b, int pw, int lane) {
+__device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *s_fmx, const int *cb_cur,
+                                  int *cb_next, int b, int nb, int pw, int lane_in) {
+  // lane-derived indices are the same every call: an opaque copy of the lane
+  // id keeps them recomputed here rather than held in registers across the
+  // batch loop
+  int lane = lane_in;
+  asm volatile("" : "+v"(lane));
   using S = WsShape<N>;
-  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, LS = S::LS, EB = S::EB, NPW = S::NPW, EPW = S::EPW;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, EPW = S::EPW, NFF4 = S::NFF4;
   const int K = a.K;
   const size_t KNp = (size_t)K * Np;
-  const int e0 = b * EB;
-  const int ne = min(EB, K - e0);
+  const int e0 = b * EB + EPW * pw;           // first element of this half
+  const int ne = max(0, min(EPW, K - e0));    // live elements of this half
+  const int el = EPW * pw;                    // first slot column of this half
 
-  for (int idx = lane; idx < EPW * GEO_STRIDE; idx += 64) {
-    const int q = idx / GEO_STRIDE, c = idx - q * GEO_STRIDE, e = pw + NPW * q;
-    const double gv = e < ne ? a.geo[(size_t)(e0 + e) * GEO_STRIDE + c] : 0.0;
-    if (c < 9)
-      sl.met[e * 9 + c] = gv;
-    else
-      sl.fgeo[e * 16 + c - 9] = gv;
+  // ---- issue every global access of the batch ----
+  constexpr int NIU = (EPW * Np * 8 + 1023) / 1024;
+  constexpr int NIG = (EPW * GEO_STRIDE * 8 + 1023) / 1024;
+#pragma unroll
+  for (int v = 0; v < 5; v++) dma_copy<NIU>(a.u_in + v * KNp + (size_t)e0 * Np, &sl.u[v][el * Np], ne * Np * 8, lane);
+  dma_copy<NIG>(a.geo + (size_t)e0 * GEO_STRIDE, &sl.geo[el * GEO_STRIDE], ne * GEO_STRIDE * 8, lane);
+
+  // wave speeds for tau (lanes 0..31: element el + lane/4, face lane%4)
+  double lam_own = 0.0, lam_nbr = 0.0;
+  if (lane < EPW * 4 && lane < ne * 4) {
+    const int cc = cb_cur[el * 4 + lane];
+    lam_own = a.maxv[(size_t)e0 * 4 + lane];
+    lam_nbr = a.maxv[(size_t)conn_nbr(cc) * 4 + conn_face(cc)];
   }
-  for (int idx = lane; idx < EPW * 4; idx += 64) {
-    const int q = idx >> 2, f = idx & 3, e = pw + NPW * q;
-    int cc = 0;
-    double tau = 0.0;
-    if (e < ne) {
-      cc = a.conn[(size_t)(e0 + e) * 4 + f];
-      tau = 0.25 * (a.maxv[(size_t)(e0 + e) * 4 + f] + a.maxv[(size_t)conn_nbr(cc) * 4 + conn_face(cc)]) *
-            a.tau_scale;
+  // connectivity of the next batch this wave stages
+  int cn = 0;
+  const int ne_n = nb >= 0 ? max(0, min(EPW, K - (nb * EB + EPW * pw))) : 0;
+  if (lane < ne_n * 4) cn = a.conn[(size_t)(nb * EB + EPW * pw) * 4 + lane];
+
+  // exterior traces: pair P = lane + 64 s is (element el + P / NFF4, face node P % NFF4)
+  const double *ubase[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) ubase[v] = a.u_in + v * KNp;
+  constexpr int NSP = EPW * NFF4;
+  constexpr int NRS = (NSP + 63) / 64;
+  double up[NRS][5];
+#pragma unroll
+  for (int s = 0; s < NRS; s++) {
+    const int P = lane + 64 * s;
+    const int q = P / NFF4, m = P - q * NFF4;
+    const bool ok = P < NSP && q < ne && m < NFF;
+    int pidx = 0;  // K*Np < 2^31 (checked on the host)
+    if (ok) {
+      const int f = m / Nfp, jf = m - f * Nfp;
+      const int cc = cb_cur[(el + q) * 4 + f];
+      pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
     }
-    sl.conn[e * 4 + f] = cc;
-    sl.tau[e * 4 + f] = tau;
+#pragma unroll
+    for (int v = 0; v < 5; v++) up[s][v] = ok ? ubase[v][pidx] : (v == 0 ? 1.0 : 0.0);
   }
 
-  // Node values (all loads issued before any use) and primitives.
-  constexpr int NPAIR = EPW * Np;
-  constexpr int NR = (NPAIR + 63) / 64;
-  double uv[NR][5];
-#pragma unroll
-  for (int s = 0; s < NR; s++) {
-    const int P = lane + 64 * s;
-    const int q = P / Np, j = P - q * Np, e = pw + NPW * q;
-    const bool ok = P < NPAIR && e < ne;
-    const size_t gi = (size_t)(e0 + e) * Np + j;
-#pragma unroll
-    for (int v = 0; v < 5; v++) uv[s][v] = ok ? a.u_in[v * KNp + gi] : (v == 0 ? 1.0 : 0.0);
+  // ---- all loads and DMA copies have landed ----
+  __builtin_amdgcn_s_waitcnt(0);
+  if (lane < EPW * 4) {
+    sl.tau[el * 4 + lane] = 0.25 * (lam_own + lam_nbr) * a.tau_scale;
+    cb_next[el * 4 + lane] = cn;
   }
+  // columns past the last element: a harmless state (rho = 1, at rest)
+  for (int i = ne * Np + lane; i < EPW * Np; i += 64) {
 #pragma unroll
-  for (int s = 0; s < NR; s++) {
-    const int P = lane + 64 * s;
-    if (P < NPAIR) {
-      const int q = P / Np, j = P - q * Np, e = pw + NPW * q;
-      const int o = j * LS + e;
-      const double r = uv[s][0], mx = uv[s][1], my = uv[s][2], mz = uv[s][3], E = uv[s][4];
-      const double inv = 1.0 / r;
-      sl.u[0][o] = r;
-      sl.u[1][o] = mx;
-      sl.u[2][o] = my;
-      sl.u[3][o] = mz;
-      sl.u[4][o] = E;
-      sl.inv[o] = inv;
-      sl.p[o] = (a.gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
-    }
+    for (int v = 0; v < 5; v++) sl.u[v][el * Np + i] = v == 0 ? 1.0 : 0.0;
   }
+  for (int i = ne * GEO_STRIDE + lane; i < EPW * GEO_STRIDE; i += 64) sl.geo[el * GEO_STRIDE + i] = 0.0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
+  // primitives
+  for (int i = lane; i < EPW * Np; i += 64) {
+    const int o = el * Np + i;
+    const double r = sl.u[0][o], mx = sl.u[1][o], my = sl.u[2][o], mz = sl.u[3][o], E = sl.u[4][o];
+    const double inv = 1.0 / r;
+    sl.inv[o] = inv;
+    sl.p[o] = (a.gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
+  }
+
   // LM (or Rusanov) surface flux of the five variables at each face node,
-  // times Fscale.  Exterior traces are gathered CH pairs per lane at a time.
-  constexpr int NSP = EPW * S::NFF4;
-  constexpr int NRS = (NSP + 63) / 64;
-  constexpr int CH = 4;
-#pragma unroll 1
-  for (int s0 = 0; s0 < NRS; s0 += CH) {
-    double up[CH][5];
+  // times Fscale
 #pragma unroll
-    for (int c = 0; c < CH; c++) {
-      const int P = lane + 64 * (s0 + c);
-      const int q = P / S::NFF4, m = P - q * S::NFF4, e = pw + NPW * q;
-      const bool ok = s0 + c < NRS && P < NSP && e < ne && m < NFF;
-      size_t pidx = 0;
-      if (ok) {
+  for (int s = 0; s < NRS; s++) {
+    const int P = lane + 64 * s;
+    if (P < NSP) {
+      const int q = P / NFF4, m = P - q * NFF4, e = el + q;
+      double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (q < ne && m < NFF) {
         const int f = m / Nfp, jf = m - f * Nfp;
-        const int cc = sl.conn[e * 4 + f];
-        pidx = (size_t)conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
-      }
+        const int o = e * Np + s_fmx[(f * 4 + 3) * Nfp + jf];
+        const double *g = &sl.geo[e * GEO_STRIDE + 9 + f * 4];
+        double uM[5];
 #pragma unroll
-      for (int v = 0; v < 5; v++) up[c][v] = ok ? a.u_in[v * KNp + pidx] : (v == 0 ? 1.0 : 0.0);
-    }
+        for (int v = 0; v < 5; v++) uM[v] = sl.u[v][o];
+        if (a.flux == 1) {
 #pragma unroll
-    for (int c = 0; c < CH; c++) {
-      const int P = lane + 64 * (s0 + c);
-      if (s0 + c < NRS && P < NSP) {
-        const int q = P / S::NFF4, m = P - q * S::NFF4, e = pw + NPW * q;
-        double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (e < ne && m < NFF) {
-          const int f = m / Nfp, jf = m - f * Nfp;
-          const int o = s_fmx[(f * 4 + 3) * Nfp + jf] * LS + e;
-          const double *g = &sl.fgeo[e * 16 + f * 4];
-          double uM[5];
+          for (int v = 0; v < 5; v++) fl[v] = euler_rusanov(v, uM, up[s], g, a.gamma) * g[3];
+        } else {
+          const double tau = sl.tau[e * 4 + f];
+          double ub[5];
 #pragma unroll
-          for (int v = 0; v < 5; v++) uM[v] = sl.u[v][o];
-          if (a.flux == 1) {
+          for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + up[s][v]);
+          const double inv = 1.0 / ub[0];
+          const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
+          const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
+          double nF[5];
+          nF[0] = g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3];
+          nF[1] = ub[1] * vn + g[0] * p;
+          nF[2] = ub[2] * vn + g[1] * p;
+          nF[3] = ub[3] * vn + g[2] * p;
+          nF[4] = (ub[4] + p) * vn;
 #pragma unroll
-            for (int v = 0; v < 5; v++) fl[v] = euler_rusanov(v, uM, up[c], g, a.gamma) * g[3];
-          } else {
-            const double tau = sl.tau[e * 4 + f];
-            double ub[5];
-#pragma unroll
-            for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + up[c][v]);
-            const double inv = 1.0 / ub[0];
-            const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
-            const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
-            double nF[5];
-            nF[0] = g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3];
-            nF[1] = ub[1] * vn + g[0] * p;
-            nF[2] = ub[2] * vn + g[1] * p;
-            nF[3] = ub[3] * vn + g[2] * p;
-            nF[4] = (ub[4] + p) * vn;
-#pragma unroll
-            for (int v = 0; v < 5; v++) fl[v] = (nF[v] - tau * (ub[v] - uM[v])) * g[3];
-          }
+          for (int v = 0; v < 5; v++) fl[v] = (nF[v] - tau * (ub[v] - uM[v])) * g[3];
         }
-#pragma unroll
-        for (int v = 0; v < 5; v++) sl.fs[v][m * EB + e] = fl[v];
       }
+#pragma unroll
+      for (int v = 0; v < 5; v++) sl.fs[v][m * EB + e] = fl[v];
     }
+    __builtin_amdgcn_sched_barrier(0);  // one face node at a time: bounds live registers
   }
 }
 
@@ -240,7 +272,7 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
                                   int lane) {
   using S = WsShape<N>;
   using P = PrecTraits<PREC_F64>;
-  constexpr int Np = S::Np, LS = S::LS, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV;
+  constexpr int Np = S::Np, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV;
   constexpr int KS = S::KS, KSS = S::KSS, KDP = S::KDP, RTF = S::RTF;
   constexpr int REM = WREM ? S::REM : 0;
   constexpr int RA = REM > 0 ? REM : 1;
@@ -272,7 +304,7 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
     }
     double g9[9];
 #pragma unroll
-    for (int c = 0; c < 9; c++) g9[c] = s.met[e * 9 + c];
+    for (int c = 0; c < 9; c++) g9[c] = s.geo[e * GEO_STRIDE + c];
     // The remainder-row operator values are the same every batch; an opaque
     // zero offset keeps them as LDS reads inside the loop instead of
     // REM*KS registers held across it.
@@ -291,7 +323,7 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
       const int j = 4 * g + kk;
       double f0 = 0.0, f1 = 0.0, f2 = 0.0;
       if (j < Np) {
-        const int o = j * LS + e;
+        const int o = e * Np + j;
         const double mx = s.u[1][o], my = s.u[2][o], mz = s.u[3][o];
         double F, Gf, H;
         if (v == 0) {
@@ -350,7 +382,7 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
         } else {
           const double kn = a.rka * kp[r] + dt * rhs;
           a.k[gi] = kn;
-          a.u_out[gi] = s.u[v][i * LS + e] + a.rkb * kn;
+          a.u_out[gi] = s.u[v][e * Np + i] + a.rkb * kn;
         }
       }
     }
@@ -376,7 +408,7 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
           } else {
             const double kn = a.rka * kpr[x] + dt * rhs;
             a.k[gi] = kn;
-            a.u_out[gi] = s.u[v][i * LS + e] + a.rkb * kn;
+            a.u_out[gi] = s.u[v][e * Np + i] + a.rkb * kn;
           }
         }
       }
@@ -398,6 +430,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   __shared__ WsSlot<N> sl[2];
   __shared__ double s_arem[REM > 0 ? REM * KDP : 1];
   __shared__ int s_fmx[16 * Nfp];
+  __shared__ int s_cb[2][S::EB * 4];  // packed connectivity of the batch being staged / the next one
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -420,14 +453,23 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   const int lo = (int)((long)nbatch * grp / ngrp), hi = (int)((long)nbatch * (grp + 1) / ngrp);
   int b = lo + gsl;
 
-  __syncthreads();  // s_fmx
-  if (w >= NC && b < hi) ws_produce<N>(a, sl[0], s_fmx, b, w - NC, lane);
+  if (w >= NC && b < hi) {
+    // connectivity of the first batch (later batches' is prefetched)
+    const int pw = w - NC, e0 = b * S::EB + S::EPW * pw;
+    if (lane < S::EPW * 4 && e0 * 4 + lane < a.K * 4) s_cb[0][S::EPW * pw * 4 + lane] = a.conn[(size_t)e0 * 4 + lane];
+  }
+  __syncthreads();  // s_fmx, s_cb
+  if (w >= NC && b < hi) ws_produce<N>(a, sl[0], s_fmx, s_cb[0], s_cb[1], b, b + gsz < hi ? b + gsz : -1, w - NC, lane);
   __syncthreads();
 
   if (w >= NC) {
-    // producer waves: stage the batch after the one being consumed
+    // producer waves: stage the batch after the one being consumed; staging
+    // call n (n = it + 1) reads the connectivity buffer n & 1
     for (int it = 0; b < hi; b += gsz, it++) {
-      if (b + gsz < hi) ws_produce<N>(a, sl[(it + 1) & 1], s_fmx, b + gsz, w - NC, lane);
+      const int nb = b + gsz, nnb = b + 2 * gsz;
+      if (nb < hi)
+        ws_produce<N>(a, sl[(it + 1) & 1], s_fmx, s_cb[(it + 1) & 1], s_cb[it & 1], nb, nnb < hi ? nnb : -1, w - NC,
+                      lane);
       __syncthreads();
     }
     return;

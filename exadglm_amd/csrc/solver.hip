@@ -346,7 +346,7 @@ bool dispatch(const dglm_ctx *c, ws_fn &ws, st_fn &st0, st_fn &st1) {
 }
 
 StageArgs stage_args(dglm_ctx *c, int s) {
-  StageArgs a;
+  StageArgs a{};
   a.K = c->K;
   a.u_in = c->u[c->cur];
   a.u_out = c->u[c->cur ^ 1];
@@ -363,6 +363,7 @@ StageArgs stage_args(dglm_ctx *c, int s) {
   a.rkb = s >= 0 ? RKB[s] : 0.0;
   a.gamma = c->p.gamma;
   a.tau_scale = c->p.tau_scale;
+  a.flux = c->p.flux == DGLM_FLUX_RUSANOV ? 1 : 0;
   return a;
 }
 
@@ -541,7 +542,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   const void *afrag_src = afr.empty() ? (const void *)afr32.data() : (const void *)afr.data();
   // Operator rows for the warp-specialised FP64 Euler kernel.
   std::vector<double> afr_ws, arem_ws;
-  if (p->physics == DGLM_EULER && p->precision == DGLM_FP64 && ws_built(p->N)) {
+  // (its gathers index one variable's nodes with 32-bit offsets)
+  if (p->physics == DGLM_EULER && p->precision == DGLM_FP64 && ws_built(p->N) && (long long)K * Np < (1LL << 31)) {
     if (p->N == 3) {
       afr_ws.resize((size_t)WsShape<3>::RTF * WsShape<3>::KS * 64);
       arem_ws.resize((size_t)WsShape<3>::REM * WsShape<3>::KDP);
