@@ -1,15 +1,20 @@
 """Benchmark: DOF-stage updates/s of the explicit DGLM RHS + LSRK4(5) step.
 
-Workload (one GPU): compressible Euler, isentropic vortex (gamma 1.4, strength 5,
-centre (5,5), free stream (1,1,1,0,1)) on [0,10]^2 x [0,10/3] with 32^3 hexes x
-6 tets = 196,608 tetrahedra, order N=4 (Np=35), LM flux, LSRK4(5) at CFL 0.4,
-FP64.  value = K * Np * 5 vars * 5 stages * steps / wall time, state resident
-in HBM (mesh, operators and initial state uploaded before the timed region).
+Headline workload (one GPU): compressible Euler, isentropic vortex (gamma 1.4,
+strength 5, centre (5,5), free stream (1,1,1,0,1)) on [0,10]^2 x [0,10/3] with
+32^3 hexes x 6 tets = 196,608 tetrahedra, order N=4 (Np=35), Rusanov flux,
+LSRK4(5) at CFL 0.4, 200 steps, FP64.  value = K * Np * 5 vars * 5 stages *
+steps / wall time, state resident in HBM (mesh, operators and initial state
+uploaded before the timed region, the step loop replayed as hipGraphs).
 
-Prints one JSON line with the roofline of the dominant kernel (the fused
-stage kernel, timed with HIP events on the solver's stream) and a CPU baseline
-(the C restatement of the reference CPU path on this host's cores, bounded
-sample).
+The same JSON line carries the roofline of the dominant kernel (the fused
+stage kernel, launch durations from HIP events on the solver's stream), a CPU
+baseline (the C restatement of the reference CPU path on this host's cores,
+bounded sample) and, unless --no-extra, the secondary figures of the metric:
+the same workload with the reference's LM flux; FP64 vs mixed precision on the
+N=6 vortex (24^3 hexes, 500 steps) with per-variable error against the FP64
+run and the analytic vortex; and the adjoint/forward time ratio of the
+Euler adjoint (vortex + Gaussian bumps, 300 steps, ~sqrt(n) checkpoints).
 """
 import argparse
 import json
@@ -64,8 +69,110 @@ def cpu_baseline(pb, u0, seconds_target=15.0):
     wall = time.perf_counter() - t0
     dof = pb.K * pb.Np * 5
     return {"value": dof * 5 * n / wall, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
-            "sample": f"{n} LSRK4(5) steps of the same workload (K={pb.K}, N={pb.N}) by oracle/dglm_oracle.c, "
+            "sample": f"{n} LSRK4(5) steps of the same mesh and state (K={pb.K}, N={pb.N}, the reference LM flux) by oracle/dglm_oracle.c, "
                       f"OpenMP {threads} threads, after 1 warm-up step"}
+
+
+def run_forward(pb, u0, precision="fp64", flux="rusanov", steps=200, warmup=10, kernel_steps=20,
+                stage_kernel="auto"):
+    """Warm up, time `steps` graph-replayed steps, then time `kernel_steps`
+    eager steps with per-launch HIP events."""
+    from exadglm_amd.solver import Solver
+
+    s = Solver(pb, "euler", cfl=0.4, tmax=math.inf, precision=precision, flux=flux)
+    if precision == "fp64":
+        s.set_stage_kernel(stage_kernel)
+    name = s.stage_kernel()
+    s.set_state(u0)
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    t, dt, ts, _ = s.time()  # raises on a non-finite dt
+    u = s.get_state()
+    if not np.isfinite(u).all():
+        raise RuntimeError("state is not finite after the timed steps")
+    tm = None
+    if kernel_steps:
+        s.set_timing(True)
+        s.step(kernel_steps)
+        s.sync()
+        tm = s.timing()
+    s.close()
+    return {"wall": wall, "u": u, "t": t, "kernel": name, "timing": tm}
+
+
+def per_var_err(u, ref):
+    """Relative L2 per conserved variable; rho*w (zero in the exact z-extruded
+    vortex) is measured against the norm of the whole momentum."""
+    e = [float(np.linalg.norm(u[v] - ref[v]) / np.linalg.norm(ref[v])) for v in range(5)]
+    e[3] = float(np.linalg.norm(u[3] - ref[3]) / np.linalg.norm(ref[1:4]))
+    return e
+
+
+def mixed_precision_leg(order=6, nhex=24, steps=490, warmup=10):
+    from exadglm_amd.problem import isentropic_vortex, uniform_problem, vortex_box
+
+    pb = uniform_problem(order, nhex + 1, nhex + 1, nhex + 1, box=vortex_box(nhex))
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    dof_stage = pb.K * pb.Np * 5 * 5 * steps
+    r64 = run_forward(pb, u0, "fp64", "rusanov", steps, warmup, 0)
+    r32 = run_forward(pb, u0, "fp32", "rusanov", steps, warmup, 0)
+    exact = isentropic_vortex(a["px"], a["py"], a["pz"], t=r64["t"])
+    return {
+        "workload": f"euler_isentropic_vortex_N{order}_{nhex}cube_hexes_x6_K{pb.K}_rusanov_{warmup + steps}steps",
+        "fp64": {"value": dof_stage / r64["wall"], "ms_per_step": r64["wall"] / steps * 1e3, "kernel": r64["kernel"]},
+        "mixed": {"value": dof_stage / r32["wall"], "ms_per_step": r32["wall"] / steps * 1e3, "kernel": r32["kernel"],
+                  "contraction": "FP32 MFMA on per-block FP64-offset fluxes, FP32 accumulation; state, pressure, "
+                                 "wave speed, offsets and sums FP64"},
+        "speedup": r64["wall"] / r32["wall"],
+        "rel_l2_mixed_vs_fp64": per_var_err(r32["u"], r64["u"]),
+        "rel_l2_fp64_vs_analytic": per_var_err(r64["u"], exact),
+        "rel_l2_mixed_vs_analytic": per_var_err(r32["u"], exact),
+        "unit": "DOF-stage updates/s",
+        "note": "rho*w errors relative to the momentum norm; target rel L2 <= 1e-8 at >= 1.8x",
+    }
+
+
+def adjoint_leg(precision="fp64", nhex=32, order=4, steps=300):
+    """Euler adjoint: J = int 1/2 rho |v|^2 at T; forward wall time (graph
+    replay) against checkpointed recording + reverse sweep."""
+    from exadglm_amd.solver import Solver
+    from exadglm_amd.workloads import c5, kinetic_energy, perturbed_state
+
+    w = c5(nhex, order)
+    pb = w["problem"]
+    u0, _ = perturbed_state(w["base"], w["bump"])
+    every = w["every"]
+    s = Solver(pb, "euler", cfl=0.4, tmax=math.inf, precision=precision, flux="lm")
+    s.set_state(u0)
+    s.step(2)
+    s.set_state(u0)
+    s.set_time(0.0, 0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    t_fwd = time.perf_counter() - t0
+    s.set_state(u0)
+    s.set_time(0.0, 0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.adjoint_record(steps, every)
+    s.sync()
+    t_rec = time.perf_counter() - t0
+    uT = s.get_state()
+    J, g = kinetic_energy(pb, uT)
+    t0 = time.perf_counter()
+    grad = s.adjoint_backward(g)
+    t_bwd = time.perf_counter() - t0
+    tape = s.tape_bytes()
+    s.close()
+    return {"J": J, "grad": grad, "t_forward_s": t_fwd, "t_record_s": t_rec, "t_backward_s": t_bwd,
+            "tape_bytes": tape, "K": pb.K, "every": every}
 
 
 def main():
@@ -75,15 +182,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nhex", type=int, default=32)
     ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--flux", default="rusanov", choices=["rusanov", "lm"],
+                    help="headline flux (the workload's Rusanov flux; lm = the reference's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the LM-flux, mixed-precision and adjoint legs")
     ap.add_argument("--kernel-steps", type=int, default=20,
                     help="eager steps after the timed region with per-launch HIP event timing (roofline)")
     ap.add_argument("--stage-kernel", default="auto", choices=["auto", "simple", "mfma", "mfma_ws"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
-                    help="contraction arithmetic (fp32: FP32 MFMA on offset-split fluxes, FP64 state)")
-    ap.add_argument("--compare", action="store_true",
-                    help="also run the FP64 path for the same steps and report the per-variable relative L2 "
-                         "difference of the final state")
+                    help="contraction precision of the headline run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a profiled run to bytes)")
@@ -105,7 +213,6 @@ def main():
         sys.exit(2)
 
     from exadglm_amd.problem import isentropic_vortex, uniform_problem, vortex_box
-    from exadglm_amd.solver import Solver
 
     n1 = args.nhex + 1
     pb = uniform_problem(args.order, n1, n1, n1, box=vortex_box(args.nhex))
@@ -113,31 +220,9 @@ def main():
     u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
     K, Np, Nfp, nvar = pb.K, pb.Np, pb.Nfp, 5
 
-    s = Solver(pb, "euler", cfl=0.4, tmax=math.inf, precision=args.precision)
-    if args.precision == "fp64":
-        s.set_stage_kernel(args.stage_kernel)
-    kernel_name = s.stage_kernel()
-    s.set_state(u0)
-    s.step(args.warmup)
-    s.sync()
-    # Timed region: the step loop as a user runs it (two-step graph replay,
-    # no events), inputs resident in HBM.
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    s.sync()
-    wall = time.perf_counter() - t0
-    t, dt, ts, _ = s.time()  # raises on a non-finite dt
-    u = s.get_state()
-    if not np.isfinite(u).all():
-        raise RuntimeError("state is not finite after the timed steps")
-    # Per-kernel launch durations: HIP events around each launch on the
-    # solver's stream, over a further run of eager steps.
-    s.set_timing(True)
-    s.step(args.kernel_steps)
-    s.sync()
-    tm = s.timing()
-    s.close()
-
+    r = run_forward(pb, u0, args.precision, args.flux, args.steps, args.warmup, args.kernel_steps,
+                    args.stage_kernel)
+    wall, tm, kernel_name = r["wall"], r["timing"], r["kernel"]
     dof_stage = K * Np * nvar * 5 * args.steps
     value = dof_stage / wall
     st_ms, st_n = tm["stage"]
@@ -146,21 +231,18 @@ def main():
     F = stage_flops(K, Np, Nfp, nvar)
     gbs = B / st_avg_s / 1e9
     tfl = F / st_avg_s / 1e12
+    pk = FP32_PEAK_TFLOPS if args.precision == "fp32" else FP64_PEAK_TFLOPS
     t_hbm = B / (HBM_PEAK_GBS * 1e9)
-    t_fp = F / (FP64_PEAK_TFLOPS * 1e12)
-    if args.precision == "fp32":
-        t_fp = F / (FP32_PEAK_TFLOPS * 1e12)
+    t_fp = F / (pk * 1e12)
     if t_hbm >= t_fp:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     else:
-        pk = FP32_PEAK_TFLOPS if args.precision == "fp32" else FP64_PEAK_TFLOPS
         roof = {"bound": "mfma", "achieved": tfl, "peak": pk, "unit": "TFLOP/s", "frac": tfl / pk}
     roof["traffic"] = None
     roof["kernel"] = f"stage_{kernel_name}_kernel<{args.order},EULER,0>"
     roof["avg_launch_us"] = st_avg_s * 1e6
     roof["algorithmic_bytes_per_launch"] = B
     roof["contraction_flops_per_launch"] = F
-    pk = FP32_PEAK_TFLOPS if args.precision == "fp32" else FP64_PEAK_TFLOPS
     roof["other"] = ({"unit": "TFLOP/s", "achieved": tfl, "peak": pk, "frac": tfl / pk}
                      if roof["bound"] == "hbm" else
                      {"unit": "GB/s", "achieved": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS})
@@ -168,7 +250,7 @@ def main():
     roof["frac_of_binding_roof"] = max(t_hbm, t_fp) / st_avg_s
 
     out = {
-        "metric": f"DOF-stage updates/s ({args.precision.upper()}, Euler N={args.order} K={K})",
+        "metric": "DOF-stage updates/s: FP64 vs mixed @L2<=1e-8; adjoint/forward time",
         "value": value,
         "unit": "DOF-stage updates/s",
         "n_gpus": 1,
@@ -180,23 +262,36 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 contraction, f64 state",
         "data": "synthetic",
-        "config": {"workload": f"euler_isentropic_vortex_N{args.order}_{args.nhex}cube_hexes_x6_K{K}_{args.precision}",
+        "config": {"workload": f"euler_isentropic_vortex_N{args.order}_{args.nhex}cube_hexes_x6_K{K}_"
+                               f"{args.flux}_{args.precision}",
                    "K": K, "N": args.order, "Np": Np, "nvar": nvar, "cfl": 0.4, "precision": args.precision,
-                   "flux": "LM", "stepper": "LSRK4(5)"},
+                   "flux": args.flux, "stepper": "LSRK4(5)"},
         "roofline": roof,
         "kernel_ms": {k: {"total_ms": v[0], "launches": v[1]} for k, v in tm.items()},
     }
-    if args.compare:
-        ref = Solver(pb, "euler", cfl=0.4, tmax=math.inf, precision="fp64")
-        ref.set_state(u0)
-        ref.step(args.warmup + args.steps)
-        ur = ref.get_state()
-        ref.close()
-        errs = [float(np.linalg.norm(u[v] - ur[v]) / np.linalg.norm(ur[v])) for v in range(5)]
-        # rho*w is zero in the exact z-extruded vortex: measure it against the momentum norm
-        errs[3] = float(np.linalg.norm(u[3] - ur[3]) / np.linalg.norm(ur[1:4]))
-        out["accuracy_vs_fp64"] = {"rel_l2_per_var": errs, "steps": args.warmup + args.steps,
-                                   "note": "rho*w relative to the momentum norm"}
+    if not args.no_extra:
+        other = "lm" if args.flux == "rusanov" else "rusanov"
+        r2 = run_forward(pb, u0, args.precision, other, args.steps, args.warmup, args.kernel_steps,
+                         args.stage_kernel)
+        st2 = r2["timing"]["stage"]
+        out[f"{other}_flux"] = {"value": dof_stage / r2["wall"], "ms_per_step": r2["wall"] / args.steps * 1e3,
+                                "stage_avg_us": st2[0] / st2[1] * 1e3, "kernel": r2["kernel"]}
+        del r, r2
+        out["mixed_precision"] = mixed_precision_leg()
+        adj = {}
+        grads = {}
+        for prec in ("fp64", "fp32"):
+            res = adjoint_leg(prec)
+            grads[prec] = res.pop("grad")
+            res["ratio_backward_over_forward"] = res["t_backward_s"] / res["t_forward_s"]
+            res["ratio_record_plus_backward_over_forward"] = (res["t_record_s"] + res["t_backward_s"]) / \
+                res["t_forward_s"]
+            adj[f"forward_{prec}"] = res
+        adj["grad_rel_diff_fp32_forward_vs_fp64"] = float(np.linalg.norm(grads["fp32"] - grads["fp64"]) /
+                                                          np.linalg.norm(grads["fp64"]))
+        adj["workload"] = "euler_adjoint_vortex_plus_1e-3_gaussian_bumps_N4_32cube_K196608_300steps_" \
+                          "J=int(rho|v|^2/2)(T), adjoint FP64"
+        out["adjoint"] = adj
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(pb, u0)
     print(json.dumps(out), flush=True)

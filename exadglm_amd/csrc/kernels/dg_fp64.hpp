@@ -70,6 +70,30 @@ __device__ inline double euler_rusanov(int v, const double uM[5], const double u
   return 0.5 * (fm + fp) + 0.5 * fmax(lm, lp) * (uM[v] - uP[v]);
 }
 
+// All five components of euler_rusanov with the primitives of each trace
+// computed once (same arithmetic, so the same values).
+__device__ inline void euler_normal_flux5(const double U[5], const double *n, double gamma, double F[5], double &lam) {
+  const double inv = 1.0 / U[0];
+  const double p = (gamma - 1.0) * (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * inv);
+  const double vn = (n[0] * U[1] + n[1] * U[2] + n[2] * U[3]) * inv;
+  lam = fabs(vn) + sqrt(fabs(gamma * p * inv));
+  F[0] = vn * U[0];
+  F[1] = U[1] * vn + n[0] * p;
+  F[2] = U[2] * vn + n[1] * p;
+  F[3] = U[3] * vn + n[2] * p;
+  F[4] = (U[4] + p) * vn;
+}
+
+__device__ inline void euler_rusanov5(const double uM[5], const double uP[5], const double *n, double gamma,
+                                      double fl[5]) {
+  double fm[5], fp[5], lm, lp;
+  euler_normal_flux5(uM, n, gamma, fm, lm);
+  euler_normal_flux5(uP, n, gamma, fp, lp);
+  const double lam = fmax(lm, lp);
+#pragma unroll
+  for (int v = 0; v < 5; v++) fl[v] = 0.5 * (fm[v] + fp[v]) + 0.5 * lam * (uM[v] - uP[v]);
+}
+
 // --------------------------------------------------------------------------
 // Wave speed per face + block maximum of lambda*Fscale
 // --------------------------------------------------------------------------

@@ -236,8 +236,9 @@ __device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *
 #pragma unroll
         for (int v = 0; v < 5; v++) uM[v] = sl.u[v][o];
         if (a.flux == 1) {
+          euler_rusanov5(uM, up[s], g, a.gamma, fl);
 #pragma unroll
-          for (int v = 0; v < 5; v++) fl[v] = euler_rusanov(v, uM, up[s], g, a.gamma) * g[3];
+          for (int v = 0; v < 5; v++) fl[v] *= g[3];
         } else {
           const double tau = sl.tau[e * 4 + f];
           double ub[5];

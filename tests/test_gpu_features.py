@@ -55,10 +55,12 @@ def test_device_integrals_match_host(gpu):
 
 
 @pytest.mark.parametrize("physics", ["advection", "euler"])
-@pytest.mark.parametrize("variant", ["simple", "mfma"])
+@pytest.mark.parametrize("variant", ["simple", "mfma", "mfma_ws"])
 def test_rusanov_equals_lm_on_continuous_traces(gpu, physics, variant):
     """Where u- = u+ (nodal interpolant of a continuous field: shared face
     nodes coincide) both fluxes reduce to n.F(u): the RHS are equal."""
+    if variant == "mfma_ws" and physics != "euler":
+        pytest.skip("the warp-specialised kernel is Euler only")
     _say(f"rusanov continuous {physics} {variant}")
     pb = cube_problem(3, 3)
     u = smooth_state(pb, physics, seed=5)
@@ -118,3 +120,46 @@ def test_rusanov_euler_vortex_runs_and_conserves(gpu):
     assert np.isfinite(u).all()
     for v in (0, 1, 2, 4):
         assert abs(I1[v] - I0[v]) <= 1e-12 * abs(I0[v])
+
+
+@pytest.mark.parametrize("variant", ["simple", "mfma", "mfma_ws"])
+def test_rusanov_euler_surface_term(gpu, variant):
+    """Euler with the Rusanov option on a discontinuous state: the RHS
+    difference to the LM flux equals -LIFT (F*_Rusanov - F*_LM) Fscale with
+    both fluxes computed from the traces in numpy."""
+    _say(f"rusanov euler surface term {variant}")
+    pb = cube_problem(3, 3)
+    from helpers import noisy_state
+
+    u = noisy_state(pb, "euler", seed=12, eps=0.05)
+    res = {}
+    for flux in ("lm", "rusanov"):
+        with Solver(pb, "euler", flux=flux) as s:
+            s.set_stage_kernel(variant)
+            res[flux] = s.rhs(u)
+    _, _, tau = res["lm"]
+    a = pb.arrays
+    K, Np, Nfp = pb.K, pb.Np, pb.Nfp
+    g = 1.4
+    Fm = a["Fmask"].reshape(4, Nfp)
+    uM = np.stack([u[v].reshape(K, Np)[:, Fm.ravel()].reshape(K, 4, Nfp) for v in range(5)])
+    uP = np.stack([u[v][a["vmapP"].reshape(K, 4, Nfp)] for v in range(5)])
+    n = np.stack([a["nx"], a["ny"], a["nz"]])[:, :, :, None]
+
+    def nflux(U):
+        inv = 1 / U[0]
+        p = (g - 1) * (U[4] - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) * inv)
+        vn = (n[0] * U[1] + n[1] * U[2] + n[2] * U[3]) * inv
+        F = np.stack([vn * U[0], U[1] * vn + n[0] * p, U[2] * vn + n[1] * p, U[3] * vn + n[2] * p, (U[4] + p) * vn])
+        return F, np.abs(vn) + np.sqrt(np.abs(g * p * inv))
+
+    fm, lm = nflux(uM)
+    fp, lp = nflux(uP)
+    f_rus = 0.5 * (fm + fp) + 0.5 * np.maximum(lm, lp) * (uM - uP)
+    ub = 0.5 * (uM + uP)
+    fb, _ = nflux(ub)
+    f_lm = fb - tau[None, :, :, None] * (ub - uM)
+    dfl = ((f_rus - f_lm) * a["Fscale"][None, :, :, None]).reshape(5, K, 4 * Nfp)
+    expect = -(dfl @ a["LIFT"].T).reshape(5, -1)
+    diff = res["rusanov"][0] - res["lm"][0]
+    assert np.abs(diff - expect).max() < 1e-12 * np.abs(expect).max()
