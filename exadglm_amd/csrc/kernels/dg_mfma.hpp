@@ -17,14 +17,18 @@
 // Precision modes:
 //   PREC_F64  v_mfma_f64_16x16x4f64; D register r of lane l is row 4r + l/16,
 //             column l%16.
-//   PREC_F32  v_mfma_f32_16x16x4f32 on  B - c, where c is an FP64 offset per
-//             (element, variable, block) -- blocks are the three volume
-//             directions and the four faces, c the block's first row -- and
-//             the exact FP64 term  sum_b c_b * rowsum_b(A)  is added in the
-//             epilogue.  The fluxes, pressure and wave speed are FP64; only the
-//             variation inside each block is rounded to FP32.  D register r of
-//             lane l is row 4(l/16) + r, column l%16.
+//   PREC_F32  v_mfma_f32_16x16x4f32 on  B - c, where c is the least-squares
+//             affine fit of B over the nodes of each (element, variable,
+//             block) -- blocks are the three volume directions and the four
+//             faces -- computed in FP64, and the exact FP64 term
+//             sum_b sum_q c_bq (A_b V_b)_q  is added in the epilogue.  The
+//             fluxes, pressure, wave speed and fit are FP64; only the
+//             second-order remainder of each block is rounded to FP32.  D
+//             register r of lane l is row 4(l/16) + r, column l%16.
 #pragma once
+#include <cmath>
+#include <utility>
+
 #include "dg_fp64.hpp"
 
 namespace dglm {
@@ -77,16 +81,117 @@ inline void arrange_a_fragments(int Np, int KD, int RT, int KS, const double *op
       }
 }
 
-// Row sums of A over the offset blocks, rowsum[i*NBLK + b] (FP64).
-inline void block_row_sums(int Np, int Nfp, const double *opT, double *rowsum) {
-  for (int i = 0; i < Np; i++)
-    for (int b = 0; b < NBLK; b++) {
-      int k0 = b < 3 ? b * Np : 3 * Np + (b - 3) * Nfp;
-      int k1 = b < 3 ? k0 + Np : k0 + Nfp;
-      double s = 0.0;
-      for (int k = k0; k < k1; k++) s += opT[(size_t)k * Np + i];
-      rowsum[i * NBLK + b] = s;
+// Offset block of contraction row k (three volume directions, four faces).
+__host__ __device__ inline int offset_block(int k, int Np, int Nfp) {
+  return k < 3 * Np ? k / Np : 3 + (k - 3 * Np) / Nfp;
+}
+
+// Host-side tables of the affine offsets, from the node coordinates of one
+// element (any affine image of the reference nodes spans the same space):
+//   fitV[k*4 + q]   basis q at the node of row k: 1 and the coordinates
+//                   relative to the block centroid (volume blocks), or 1 and
+//                   two orthonormal in-plane coordinates (faces; q = 3 is 0);
+//   fitP[k*4 + q]   least-squares weights, c_bq = sum_{k in b} fitP[k*4+q] B_k;
+//   fitAV[(i*NBLK + b)*4 + q] = sum_{k in b} A[i][k] fitV[k*4+q]  (FP64).
+// Returns false if a block's normal matrix is singular.
+inline bool affine_fit_tables(int Np, int Nfp, const int *Fmask, const double *x, const double *y, const double *z,
+                              const double *opT, double *fitV, double *fitP, double *fitAV) {
+  const int KD = 3 * Np + 4 * Nfp;
+  for (int b = 0; b < NBLK; b++) {
+    const int k0 = b < 3 ? b * Np : 3 * Np + (b - 3) * Nfp, n = b < 3 ? Np : Nfp;
+    auto node = [&](int j) { return b < 3 ? j : Fmask[(b - 3) * Nfp + j]; };
+    double cx = 0, cy = 0, cz = 0;
+    for (int j = 0; j < n; j++) {
+      cx += x[node(j)];
+      cy += y[node(j)];
+      cz += z[node(j)];
     }
+    cx /= n;
+    cy /= n;
+    cz /= n;
+    double e1[3] = {1, 0, 0}, e2[3] = {0, 1, 0}, e3[3] = {0, 0, 1};
+    int nb = 4;
+    if (b >= 3) {
+      // orthonormal in-plane axes from the face's node spread
+      int ja = 0;
+      double best = -1;
+      for (int j = 0; j < n; j++) {
+        const double dx = x[node(j)] - cx, dy = y[node(j)] - cy, dz = z[node(j)] - cz, d = dx * dx + dy * dy + dz * dz;
+        if (d > best) best = d, ja = j;
+      }
+      double a[3] = {x[node(ja)] - cx, y[node(ja)] - cy, z[node(ja)] - cz};
+      double la = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      for (double &v : a) v /= la;
+      int jb = 0;
+      best = -1;
+      for (int j = 0; j < n; j++) {
+        double d[3] = {x[node(j)] - cx, y[node(j)] - cy, z[node(j)] - cz};
+        const double pr = d[0] * a[0] + d[1] * a[1] + d[2] * a[2];
+        for (int c = 0; c < 3; c++) d[c] -= pr * a[c];
+        const double q = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (q > best) best = q, jb = j;
+      }
+      double bvec[3] = {x[node(jb)] - cx, y[node(jb)] - cy, z[node(jb)] - cz};
+      const double pr = bvec[0] * a[0] + bvec[1] * a[1] + bvec[2] * a[2];
+      for (int c = 0; c < 3; c++) bvec[c] -= pr * a[c];
+      const double lb = std::sqrt(bvec[0] * bvec[0] + bvec[1] * bvec[1] + bvec[2] * bvec[2]);
+      for (int c = 0; c < 3; c++) {
+        e1[c] = a[c];
+        e2[c] = bvec[c] / lb;
+        e3[c] = 0.0;
+      }
+      nb = 3;
+    }
+    for (int j = 0; j < n; j++) {
+      const double d[3] = {x[node(j)] - cx, y[node(j)] - cy, z[node(j)] - cz};
+      double *V = &fitV[(size_t)(k0 + j) * 4];
+      V[0] = 1.0;
+      V[1] = d[0] * e1[0] + d[1] * e1[1] + d[2] * e1[2];
+      V[2] = d[0] * e2[0] + d[1] * e2[1] + d[2] * e2[2];
+      V[3] = nb == 4 ? d[0] * e3[0] + d[1] * e3[1] + d[2] * e3[2] : 0.0;
+    }
+    // normal matrix G = V^T V (nb x nb), inverse by Gauss-Jordan
+    double G[4][8] = {};
+    for (int p = 0; p < nb; p++) {
+      for (int q = 0; q < nb; q++) {
+        double sum = 0;
+        for (int j = 0; j < n; j++) sum += fitV[(size_t)(k0 + j) * 4 + p] * fitV[(size_t)(k0 + j) * 4 + q];
+        G[p][q] = sum;
+      }
+      G[p][4 + p] = 1.0;
+    }
+    for (int c = 0; c < nb; c++) {
+      int piv = c;
+      for (int r = c + 1; r < nb; r++)
+        if (std::fabs(G[r][c]) > std::fabs(G[piv][c])) piv = r;
+      if (std::fabs(G[piv][c]) < 1e-300) return false;
+      for (int q = 0; q < 8; q++) std::swap(G[c][q], G[piv][q]);
+      const double inv = 1.0 / G[c][c];
+      for (int q = 0; q < 8; q++) G[c][q] *= inv;
+      for (int r = 0; r < nb; r++)
+        if (r != c) {
+          const double f = G[r][c];
+          for (int q = 0; q < 8; q++) G[r][q] -= f * G[c][q];
+        }
+    }
+    for (int j = 0; j < n; j++)
+      for (int q = 0; q < 4; q++) {
+        double w = 0;
+        if (q < nb)
+          for (int p = 0; p < nb; p++) w += G[q][4 + p] * fitV[(size_t)(k0 + j) * 4 + p];
+        fitP[(size_t)(k0 + j) * 4 + q] = w;
+      }
+  }
+  for (int i = 0; i < Np; i++)
+    for (int b = 0; b < NBLK; b++)
+      for (int q = 0; q < 4; q++) {
+        const int k0 = b < 3 ? b * Np : 3 * Np + (b - 3) * Nfp, n = b < 3 ? Np : Nfp;
+        double sum = 0;
+        for (int k = k0; k < k0 + n; k++) sum += opT[(size_t)k * Np + i] * fitV[(size_t)k * 4 + q];
+        fitAV[((size_t)i * NBLK + b) * 4 + q] = sum;
+      }
+  (void)KD;
+  return true;
 }
 
 // Volume flux rows of variable v at node q: (fR, fS, fT) = metric . (F, G, H).
@@ -127,7 +232,7 @@ __device__ inline void volume_flux_v(int v, const double *const u[5], const doub
 //   MODE 0: LSRK update; MODE 1: write the RHS only.
 template <int N, int PHYS, int MODE, int PREC>
 __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs a, const void *__restrict__ afrag_v,
-                                                                      const double *__restrict__ rowsum,
+                                                                      const double *__restrict__ fit,
                                                                       int nbatch) {
   using O = Ord<N>;
   using S = MfmaShape<N>;
@@ -145,7 +250,10 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
   __shared__ double s_vel[NVEL][PHYS == PHYS_ADV ? EB * Np : 1];
   __shared__ double s_geo[EB * GEO_STRIDE];
   __shared__ double s_tau[EB * 4];
-  __shared__ double s_c[OFFS ? NBLK * EB : 1];
+  __shared__ double s_b64[OFFS ? KDP * EB : 1];    // FP64 flux tile before the fit is removed
+  __shared__ double s_c[OFFS ? NBLK * 4 * EB : 1];  // affine-fit coefficients [block][q][element]
+  // fit = [fitV (KD x 4) | fitP (KD x 4) | fitAV (Np x NBLK x 4)]
+  const double *fitV = fit, *fitP = fit + 4 * KD, *fitAV = fit + 8 * KD;
   __shared__ int s_conn[EB * 4];
   __shared__ int s_fmx[16 * Nfp];
 
@@ -233,28 +341,6 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
 
 #pragma unroll 1
     for (int v = 0; v < NV; v++) {
-      if (OFFS) {
-        // Offsets: the first row of each block (node 0 of the volume blocks,
-        // face node 0 of each face) in FP64.
-        for (int idx = tid; idx < NBLK * EB; idx += NT) {
-          const int bb = idx >> 4, e = idx & 15;
-          double c = 0.0;
-          if (e < ne) {
-            if (bb < 3) {
-              const double *uu[5], *vv[3];
-              for (int q = 0; q < 5; q++) uu[q] = s_u[q < NV ? q : 0] + e * Np;
-              for (int q = 0; q < 3; q++) vv[q] = s_vel[q < NVEL ? q : 0] + e * Np;
-              double f0, f1, f2;
-              volume_flux_v<PHYS>(v, uu, vv, 0, &s_geo[e * GEO_STRIDE], a.gamma, f0, f1, f2);
-              c = bb == 0 ? f0 : bb == 1 ? f1 : f2;
-            } else {
-              c = lm_flux(v, e, (bb - 3) * Nfp);
-            }
-          }
-          s_c[bb * EB + e] = c;
-        }
-        __syncthreads();
-      }
       // ---- B_v tile: volume rows (d*Np + j), column e ----
       for (int idx = tid; idx < Np * EB; idx += NT) {
         const int j = idx >> 4, e = idx & 15;
@@ -264,27 +350,53 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
           for (int q = 0; q < 5; q++) uu[q] = s_u[q < NV ? q : 0] + e * Np;
           for (int q = 0; q < 3; q++) vv[q] = s_vel[q < NVEL ? q : 0] + e * Np;
           volume_flux_v<PHYS>(v, uu, vv, j, &s_geo[e * GEO_STRIDE], a.gamma, f0, f1, f2);
-          if (OFFS) {
-            f0 -= s_c[0 * EB + e];
-            f1 -= s_c[1 * EB + e];
-            f2 -= s_c[2 * EB + e];
-          }
         }
-        s_f[j * EB + e] = (T)f0;
-        s_f[(Np + j) * EB + e] = (T)f1;
-        s_f[(2 * Np + j) * EB + e] = (T)f2;
+        if constexpr (OFFS) {
+          s_b64[j * EB + e] = f0;
+          s_b64[(Np + j) * EB + e] = f1;
+          s_b64[(2 * Np + j) * EB + e] = f2;
+        } else {
+          s_f[j * EB + e] = f0;
+          s_f[(Np + j) * EB + e] = f1;
+          s_f[(2 * Np + j) * EB + e] = f2;
+        }
       }
       // ---- B_v tile: LM surface rows (3Np + m) ----
       for (int idx = tid; idx < NFF * EB; idx += NT) {
         const int m = idx >> 4, e = idx & 15;
         double fl = 0.0;
-        if (e < ne) {
-          fl = lm_flux(v, e, m);
-          if (OFFS) fl -= s_c[(3 + m / Nfp) * EB + e];
-        }
-        s_f[(3 * Np + m) * EB + e] = (T)fl;
+        if (e < ne) fl = lm_flux(v, e, m);
+        if constexpr (OFFS)
+          s_b64[(3 * Np + m) * EB + e] = fl;
+        else
+          s_f[(3 * Np + m) * EB + e] = fl;
       }
       __syncthreads();
+      if constexpr (OFFS) {
+        // least-squares affine fit per (block, element): c_bq = sum_k fitP[k][q] B_k
+        for (int task = tid; task < NBLK * 4 * EB; task += NT) {
+          const int bb = task / (4 * EB), q = (task / EB) & 3, e = task & 15;
+          const int k0 = bb < 3 ? bb * Np : 3 * Np + (bb - 3) * Nfp, k1 = bb < 3 ? k0 + Np : k0 + Nfp;
+          double c0 = 0.0, c1 = 0.0;
+          int k = k0;
+          for (; k + 1 < k1; k += 2) {
+            c0 += fitP[k * 4 + q] * s_b64[k * EB + e];
+            c1 += fitP[(k + 1) * 4 + q] * s_b64[(k + 1) * EB + e];
+          }
+          if (k < k1) c0 += fitP[k * 4 + q] * s_b64[k * EB + e];
+          s_c[task] = c0 + c1;
+        }
+        __syncthreads();
+        // the remainder B - fit, rounded to the MFMA precision
+        for (int idx = tid; idx < KD * EB; idx += NT) {
+          const int k = idx >> 4, e = idx & 15;
+          const double *cb = &s_c[offset_block(k, Np, Nfp) * 4 * EB + e];
+          const double *V = &fitV[k * 4];
+          const double fv = cb[0] * V[0] + cb[EB] * V[1] + cb[2 * EB] * V[2] + cb[3 * EB] * V[3];
+          s_f[idx] = (T)(s_b64[idx] - fv);
+        }
+        __syncthreads();
+      }
 
       // ---- contraction on MFMA ----
       typename P::Acc acc = {0, 0, 0, 0};
@@ -303,8 +415,10 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
           if (i < Np) {
             double rhs = (double)acc[r];
             if (OFFS) {
+              // exact FP64 contraction of the fit: sum_b sum_q c_bq (A_b V_b)_q
+              const double *av = &fitAV[(size_t)i * NBLK * 4];
 #pragma unroll
-              for (int bb = 0; bb < NBLK; bb++) rhs += s_c[bb * EB + e] * rowsum[i * NBLK + bb];
+              for (int bq = 0; bq < NBLK * 4; bq++) rhs += s_c[bq * EB + e] * av[bq];
             }
             const size_t gi = v * KNp + (size_t)(e0 + e) * Np + i;
             if (MODE == 1) {

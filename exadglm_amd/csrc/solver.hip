@@ -67,7 +67,7 @@ struct dglm_ctx {
   int nblk_ws = 0;
   int variant = DGLM_STAGE_AUTO;  // requested stage kernel
   void *afrag = nullptr;          // operator A-fragments for the MFMA kernel (FP64 or FP32)
-  double *rowsum = nullptr;       // FP64 block row sums of the operator (mixed precision)
+  double *rowsum = nullptr;       // FP64 affine-offset tables of the mixed-precision contraction
   int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
   double *afrag_ws = nullptr;     // operator rows in the warp-specialised kernel's k order
   double *arem_ws = nullptr;      // rows left to the VALU by that kernel
@@ -531,8 +531,17 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
     if (p->precision == DGLM_FP32) {
       afr32.resize((size_t)RT * KS * 64);
       arrange_a_fragments(Np, KD, RT, KS, op.data(), afr32.data());
-      rsum.resize((size_t)Np * NBLK);
-      block_row_sums(Np, Nfp, op.data(), rsum.data());
+      // affine-offset tables: [fitV (KD x 4) | fitP (KD x 4) | fitAV (Np x NBLK x 4)]
+      rsum.assign((size_t)8 * KD + (size_t)Np * NBLK * 4, 0.0);
+      std::vector<double> rx(Np), ry(Np), rz(Np);
+      for (int i = 0; i < Np; i++) {
+        rx[i] = ref.rst[i * 3];
+        ry[i] = ref.rst[i * 3 + 1];
+        rz[i] = ref.rst[i * 3 + 2];
+      }
+      if (!affine_fit_tables(Np, Nfp, ref.Fmask.data(), rx.data(), ry.data(), rz.data(), op.data(), rsum.data(),
+                             rsum.data() + 4 * KD, rsum.data() + 8 * KD))
+        return fail(DGLM_ERR_ARG, "dglm_create: singular affine-offset basis");
     } else {
       afr.resize((size_t)RT * KS * 64);
       arrange_a_fragments(Np, KD, RT, KS, op.data(), afr.data());
@@ -583,7 +592,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (afrag_bytes && (rc = dev_alloc((char **)&c->afrag, afrag_bytes, "A-fragments"))) ||
       (!afr_ws.empty() && (rc = dev_alloc(&c->afrag_ws, afr_ws.size(), "A-fragments (warp-specialised)"))) ||
       (!arem_ws.empty() && (rc = dev_alloc(&c->arem_ws, arem_ws.size(), "remainder rows"))) ||
-      (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "row sums")))) {
+      (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "affine-offset tables")))) {
     free_ctx(c);
     return rc;
   }
