@@ -1,36 +1,35 @@
 // Warp-specialised FP64 stage kernel for the Euler equations (gfx950), and
 // the LDS-staged wave-speed kernel.
 //
-// Stage kernel: one persistent workgroup per CU, double-buffered LDS slots of
-// 16 elements ("batches"), two wave roles that meet at one barrier a batch:
+// Stage kernel: one persistent workgroup per CU, batches of 16 elements, two
+// wave roles and two phases per batch (one barrier each):
 //
-//   producer waves (NPW = 2, one per half batch of 8 elements)  stage batch
-//       b+1 into the free slot while the consumers work on batch b.  Every
-//       global access of a batch is issued at once -- node values and the
-//       geometry records go straight from global memory into LDS
-//       (global_load_lds_dwordx4, no registers), the exterior traces of all
-//       face nodes and the wave speeds for tau are gathered into registers
-//       with the connectivity prefetched one batch earlier -- so a batch
-//       costs one memory round trip.  Then the primitives 1/rho and p, tau
-//       per face and the LM surface flux of all five variables at every face
-//       node (the exterior trace gathered once per face node, not once per
-//       variable) are written to the slot;
-//   consumer waves (5 x RTF)  wave (v, t) computes row tile t (16 nodes) of
-//       variable v for batch b: the volume-flux B-fragments are formed in
-//       registers from the slot (flux of one node reused by its three
-//       direction k-steps), the surface B-fragments are read from the slot,
-//       v_mfma_f64_16x16x4f64 accumulates against the operator rows held in
-//       registers, and the LSRK update is the epilogue.  The Np - 16*RTF
-//       rows that do not fill a tile are accumulated on the VALU by the t = 0
-//       waves (operator rows in LDS, partial sums reduced across the four
-//       k-lanes at the end) instead of padding a whole MFMA tile.
+//   phase 1  consumers contract the surface rows of batch b (the LM surface
+//            fluxes, staged in a single LDS buffer); producers issue every
+//            global access of batch b+1 at once -- node values and geometry
+//            records go straight into the batch's LDS slot
+//            (global_load_lds_dwordx4, no registers), the exterior traces of
+//            all face nodes and the wave speeds for tau are gathered into
+//            registers with the connectivity prefetched a batch earlier --
+//            then write p, the contravariant velocity V_d = sum_x g_dx v_x and
+//            tau for b+1;
+//   phase 2  consumers contract the volume rows of batch b and apply the
+//            LSRK update; producers turn b+1's traces into the LM (or
+//            Rusanov) surface flux of all five variables at every face node,
+//            written once per face node into the surface buffer that the
+//            consumers finished reading in phase 1.
 //
-// The contraction order is  k = d*Np4 + j  (direction d, node j, Np4 = Np
-// rounded up to 4) followed by the face nodes, so a lane's k-steps of one node
-// in the three directions share one flux evaluation.  Batches are assigned
-// XCD-aware: the CUs of one XCD walk a contiguous range of batches together,
-// so the faces shared between neighbouring batches are served from that
-// XCD's L2.
+// Consumer wave (v, t) owns row tile t (16 nodes) of variable v and holds its
+// operator rows as v_mfma_f64_16x16x4f64 A-fragments in registers.  Its
+// volume B-fragments are formed in registers: for variable v the metric
+// flux is  f_d = c V_d + p g_d,v-1  (c = rho, rho*u_v or E + p; the pressure
+// term only for the momenta), one node's value feeding its three direction
+// k-steps (contraction order k = d*Np4 + j).  The Np - 16*RTF rows that do not
+// fill a tile are accumulated on the VALU by the t = 0 waves (operator rows in
+// LDS, partial sums reduced across the four k-lanes with two shuffles)
+// instead of padding a whole MFMA tile.  The CUs of one XCD walk a contiguous
+// range of batches, so the traces shared by neighbouring batches are served
+// from that XCD's L2.
 //
 // Reference semantics as in dg_fp64.hpp (euler_dglm.cpp:6-302,
 // euler_time.cpp:15-38).
@@ -60,26 +59,31 @@ struct WsShape {
   static constexpr int EPW = EB / NPW;        // elements staged per producer wave (a contiguous half)
 };
 
-// One batch in LDS.  Node arrays are element-major [element][node] (the
-// global order, so a half batch is one contiguous DMA copy per variable); a
-// B-fragment read of 16 elements then strides Np doubles, which spreads the
-// 16 lanes over distinct banks for odd Np.
+// One batch's node data in LDS (two of these alternate).  Node arrays are
+// element-major [element][node] (the global order, so a half batch is one
+// contiguous DMA copy per variable); a B-fragment read of 16 elements then
+// strides Np doubles, which spreads the lanes over distinct banks for odd Np.
 template <int N>
 struct WsSlot {
   using S = WsShape<N>;
   alignas(16) double u[5][S::EB * S::Np];
   alignas(16) double geo[S::EB * GEO_STRIDE];  // 9 metrics, 4 x (nx, ny, nz, Fscale)
-  double inv[S::EB * S::Np];
   double p[S::EB * S::Np];
-  double fs[5][S::NFF4 * S::EB];  // surface rows, [face node][element]
+  double V[3][S::EB * S::Np];  // contravariant velocity
   double tau[S::EB * 4];
 };
 
 template <int N>
+constexpr size_t ws_lds_bytes() {
+  using S = WsShape<N>;
+  return 2 * sizeof(WsSlot<N>) + 8 * 5 * S::NFF4 * S::EB + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp +
+         4 * 2 * S::EB * 4;
+}
+
+template <int N>
 constexpr bool ws_available() {
   using S = WsShape<N>;
-  return S::RTF >= 1 && S::NT <= 1024 &&
-         2 * sizeof(WsSlot<N>) + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp + 4 * 2 * S::EB * 4 <= 160 * 1024;
+  return S::RTF >= 1 && S::NT <= 1024 && ws_lds_bytes<N>() <= 160 * 1024;
 }
 
 // Host-side: operator rows in the k order of the kernel.
@@ -114,12 +118,11 @@ typedef __attribute__((address_space(3))) void lds_void;
 // Copy `bytes` (a multiple of 8, at most 64*16*NI) from global src to LDS dst
 // with one global_load_lds_dwordx4 per KiB: lane l of instruction q moves the
 // 16 bytes at offset (64q + l)*16; lanes past the end are masked off (a final
-// 8-byte tail is moved by a plain load and store).  src, dst 16-byte aligned;
-// dst is wave-uniform.
+// 8-byte tail is moved by a plain load and store).  dst 16-byte aligned and
+// wave-uniform; a source that is not 16-byte aligned takes a plain copy.
 template <int NI>
 __device__ inline void dma_copy(const double *src, double *dst, int bytes, int lane) {
   if (((uintptr_t)src & 15) != 0) {
-    // source not 16-byte aligned (odd element counts): plain copy
     for (int i = lane; i < (bytes >> 3); i += 64) dst[i] = src[i];
     return;
   }
@@ -133,42 +136,47 @@ __device__ inline void dma_copy(const double *src, double *dst, int bytes, int l
   if ((bytes & 15) && lane == 0) dst[2 * n16] = src[2 * n16];
 }
 
-// Producer wave pw stages elements [8pw, 8pw+8) of batch b into slot sl.
-// cb_cur holds the batch's packed connectivity (loaded by the previous call);
-// the connectivity of batch nb (the next one this wave stages, < 0: none) is
-// loaded into cb_next.  Every LDS entry the wave reads back it wrote itself,
-// so a wave-level fence is the only synchronisation inside the producer.
 template <int N>
-__device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *s_fmx, const int *cb_cur,
-                                  int *cb_next, int b, int nb, int pw, int lane_in) {
+struct WsTraces {
+  static constexpr int NRS = (WsShape<N>::EPW * WsShape<N>::NFF4 + 63) / 64;
+  double up[NRS][5];  // exterior traces of this lane's face-node pairs
+};
+
+// Producer phase 1 for batch b (elements [8pw, 8pw+8) of the batch): issue
+// every load, then write tau, p and V into slot sl.  cb_cur holds the batch's
+// packed connectivity; that of batch nb (< 0: none) is loaded into cb_next.
+// The exterior traces stay in registers (tr) for phase 2.  Every LDS entry the
+// wave reads back it wrote itself, so a wave-level fence is the only
+// synchronisation inside the producer.
+template <int N>
+__device__ inline void ws_stage_nodes(const StageArgs &a, WsSlot<N> &sl, const int *s_fmx, const int *cb_cur,
+                                      int *cb_next, int b, int nb, int pw, int lane_in, WsTraces<N> &tr) {
+  using S = WsShape<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, EPW = S::EPW, NFF4 = S::NFF4;
+  constexpr int NRS = WsTraces<N>::NRS;
   // lane-derived indices are the same every call: an opaque copy of the lane
   // id keeps them recomputed here rather than held in registers across the
   // batch loop
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
-  using S = WsShape<N>;
-  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, EPW = S::EPW, NFF4 = S::NFF4;
   const int K = a.K;
   const size_t KNp = (size_t)K * Np;
   const int e0 = b * EB + EPW * pw;           // first element of this half
   const int ne = max(0, min(EPW, K - e0));    // live elements of this half
   const int el = EPW * pw;                    // first slot column of this half
 
-  // ---- issue every global access of the batch ----
   constexpr int NIU = (EPW * Np * 8 + 1023) / 1024;
   constexpr int NIG = (EPW * GEO_STRIDE * 8 + 1023) / 1024;
 #pragma unroll
   for (int v = 0; v < 5; v++) dma_copy<NIU>(a.u_in + v * KNp + (size_t)e0 * Np, &sl.u[v][el * Np], ne * Np * 8, lane);
   dma_copy<NIG>(a.geo + (size_t)e0 * GEO_STRIDE, &sl.geo[el * GEO_STRIDE], ne * GEO_STRIDE * 8, lane);
 
-  // wave speeds for tau (lanes 0..31: element el + lane/4, face lane%4)
   double lam_own = 0.0, lam_nbr = 0.0;
   if (lane < EPW * 4 && lane < ne * 4) {
     const int cc = cb_cur[el * 4 + lane];
     lam_own = a.maxv[(size_t)e0 * 4 + lane];
     lam_nbr = a.maxv[(size_t)conn_nbr(cc) * 4 + conn_face(cc)];
   }
-  // connectivity of the next batch this wave stages
   int cn = 0;
   const int ne_n = nb >= 0 ? max(0, min(EPW, K - (nb * EB + EPW * pw))) : 0;
   if (lane < ne_n * 4) cn = a.conn[(size_t)(nb * EB + EPW * pw) * 4 + lane];
@@ -177,14 +185,11 @@ __device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *
   const double *ubase[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) ubase[v] = a.u_in + v * KNp;
-  constexpr int NSP = EPW * NFF4;
-  constexpr int NRS = (NSP + 63) / 64;
-  double up[NRS][5];
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int P = lane + 64 * s;
     const int q = P / NFF4, m = P - q * NFF4;
-    const bool ok = P < NSP && q < ne && m < NFF;
+    const bool ok = P < EPW * NFF4 && q < ne && m < NFF;
     int pidx = 0;  // K*Np < 2^31 (checked on the host)
     if (ok) {
       const int f = m / Nfp, jf = m - f * Nfp;
@@ -192,10 +197,10 @@ __device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *
       pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) up[s][v] = ok ? ubase[v][pidx] : (v == 0 ? 1.0 : 0.0);
+    for (int v = 0; v < 5; v++) tr.up[s][v] = ok ? ubase[v][pidx] : (v == 0 ? 1.0 : 0.0);
   }
 
-  // ---- all loads and DMA copies have landed ----
+  // ---- every load and DMA copy has landed ----
   __builtin_amdgcn_s_waitcnt(0);
   if (lane < EPW * 4) {
     sl.tau[el * 4 + lane] = 0.25 * (lam_own + lam_nbr) * a.tau_scale;
@@ -211,21 +216,37 @@ __device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-  // primitives
+  // pressure and contravariant velocity
   for (int i = lane; i < EPW * Np; i += 64) {
-    const int o = el * Np + i;
+    const int o = el * Np + i, e = el + i / Np;
     const double r = sl.u[0][o], mx = sl.u[1][o], my = sl.u[2][o], mz = sl.u[3][o], E = sl.u[4][o];
     const double inv = 1.0 / r;
-    sl.inv[o] = inv;
     sl.p[o] = (a.gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
+    const double vx = mx * inv, vy = my * inv, vz = mz * inv;
+    const double *g = &sl.geo[e * GEO_STRIDE];
+#pragma unroll
+    for (int d = 0; d < 3; d++) sl.V[d][o] = g[3 * d] * vx + g[3 * d + 1] * vy + g[3 * d + 2] * vz;
   }
+}
 
-  // LM (or Rusanov) surface flux of the five variables at each face node,
-  // times Fscale
+// Producer phase 2: LM (or Rusanov) surface flux of the five variables at each
+// face node of this wave's half of batch b, times Fscale, into fs
+// ([variable][face node][element]).
+template <int N>
+__device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl, double *fs, const int *s_fmx, int b,
+                                        int pw, int lane_in, const WsTraces<N> &tr) {
+  using S = WsShape<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, EPW = S::EPW, NFF4 = S::NFF4;
+  constexpr int NRS = WsTraces<N>::NRS;
+  int lane = lane_in;
+  asm volatile("" : "+v"(lane));
+  const int e0 = b * EB + EPW * pw;
+  const int ne = max(0, min(EPW, a.K - e0));
+  const int el = EPW * pw;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int P = lane + 64 * s;
-    if (P < NSP) {
+    if (P < EPW * NFF4) {
       const int q = P / NFF4, m = P - q * NFF4, e = el + q;
       double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (q < ne && m < NFF) {
@@ -236,14 +257,14 @@ __device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *
 #pragma unroll
         for (int v = 0; v < 5; v++) uM[v] = sl.u[v][o];
         if (a.flux == 1) {
-          euler_rusanov5(uM, up[s], g, a.gamma, fl);
+          euler_rusanov5(uM, tr.up[s], g, a.gamma, fl);
 #pragma unroll
           for (int v = 0; v < 5; v++) fl[v] *= g[3];
         } else {
           const double tau = sl.tau[e * 4 + f];
           double ub[5];
 #pragma unroll
-          for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + up[s][v]);
+          for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + tr.up[s][v]);
           const double inv = 1.0 / ub[0];
           const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
           const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
@@ -258,22 +279,23 @@ __device__ inline void ws_produce(const StageArgs &a, WsSlot<N> &sl, const int *
         }
       }
 #pragma unroll
-      for (int v = 0; v < 5; v++) sl.fs[v][m * EB + e] = fl[v];
+      for (int v = 0; v < 5; v++) fs[(v * NFF4 + m) * EB + e] = fl[v];
     }
     __builtin_amdgcn_sched_barrier(0);  // one face node at a time: bounds live registers
   }
 }
 
-// Consumer wave (v, t): batches b, b+gsz, ... < hi from alternating slots.
-// WREM: this wave also accumulates the REM rows beyond the full tiles (kept
-// as a separate instantiation so both loops stay straight-line code).
+// Consumer wave (v, t): batches b, b+gsz, ... < hi from alternating slots, two
+// barriers per batch.  WREM: this wave also accumulates the REM rows beyond
+// the full tiles (a separate instantiation so both loops stay straight-line
+// code).
 template <int N, int MODE, bool WREM>
-__device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const double *s_arem,
+__device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const double *fs, const double *s_arem,
                                   const double *__restrict__ afrag, double dt, int b, int hi, int gsz, int v, int t,
                                   int lane) {
   using S = WsShape<N>;
   using P = PrecTraits<PREC_F64>;
-  constexpr int Np = S::Np, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV;
+  constexpr int Np = S::Np, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV, NFF4 = S::NFF4;
   constexpr int KS = S::KS, KSS = S::KSS, KDP = S::KDP, RTF = S::RTF;
   constexpr int REM = WREM ? S::REM : 0;
   constexpr int RA = REM > 0 ? REM : 1;
@@ -282,9 +304,15 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
   const size_t KNp = (size_t)K * Np;
   const int kk = lane >> 4, e = lane & 15;
 
+  // operator rows of this wave, resident for the whole kernel (the empty asm
+  // makes them opaque, so they are never re-loaded inside the loop)
   double afr[KS];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) afr[ks] = afrag[((size_t)t * KS + ks) * 64 + lane];
+  for (int ks = 0; ks < KS; ks++) {
+    afr[ks] = afrag[((size_t)t * KS + ks) * 64 + lane];
+    asm volatile("" : "+v"(afr[ks]));
+  }
+  const double *fsv = fs + v * NFF4 * EB;
 
   for (int it = 0; b < hi; b += gsz, it++) {
     const WsSlot<N> &s = sl[it & 1];
@@ -303,9 +331,6 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
         kpr[x] = (live && rr < REM) ? a.k[gbase + 16 * RTF + rr] : 0.0;
       }
     }
-    double g9[9];
-#pragma unroll
-    for (int c = 0; c < 9; c++) g9[c] = s.geo[e * GEO_STRIDE + c];
     // The remainder-row operator values are the same every batch; an opaque
     // zero offset keeps them as LDS reads inside the loop instead of
     // REM*KS registers held across it.
@@ -318,30 +343,49 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
 #pragma unroll
     for (int r = 0; r < RA; r++) racc[r] = 0.0;
 
-    // ---- volume rows: one flux evaluation per node, three k-steps ----
+    // ---- phase 1: surface rows ----
+#pragma unroll
+    for (int ks = 0; ks < KSS; ks++) {
+      const double bf = fsv[(4 * ks + kk) * EB + e];
+      acc = P::mfma(afr[3 * G + ks], bf, acc);
+#pragma unroll
+      for (int r = 0; r < REM; r++) {
+        racc[r] += arw[r * KDP + KV + 4 * ks] * bf;
+        asm volatile("" : "+v"(racc[r]));
+      }
+      if (ks % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();  // surface buffer free for the next batch's fluxes
+
+    // ---- phase 2: volume rows, f_d = c V_d + p g_d,v-1 ----
+    double gv[3] = {0.0, 0.0, 0.0};
+    if (v >= 1 && v <= 3) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) gv[d] = s.geo[e * GEO_STRIDE + 3 * d + v - 1];
+    }
 #pragma unroll
     for (int g = 0; g < G; g++) {
       const int j = 4 * g + kk;
       double f0 = 0.0, f1 = 0.0, f2 = 0.0;
       if (j < Np) {
         const int o = e * Np + j;
-        const double mx = s.u[1][o], my = s.u[2][o], mz = s.u[3][o];
-        double F, Gf, H;
+        const double V0 = s.V[0][o], V1 = s.V[1][o], V2 = s.V[2][o];
         if (v == 0) {
-          F = mx;
-          Gf = my;
-          H = mz;
+          const double c = s.u[0][o];
+          f0 = c * V0;
+          f1 = c * V1;
+          f2 = c * V2;
+        } else if (v == 4) {
+          const double c = s.u[4][o] + s.p[o];
+          f0 = c * V0;
+          f1 = c * V1;
+          f2 = c * V2;
         } else {
-          const double inv = s.inv[o], p = s.p[o];
-          const double vx = mx * inv, vy = my * inv, vz = mz * inv;
-          const double c = v == 4 ? s.u[4][o] + p : (v == 1 ? mx : v == 2 ? my : mz);
-          F = c * vx + (v == 1 ? p : 0.0);
-          Gf = c * vy + (v == 2 ? p : 0.0);
-          H = c * vz + (v == 3 ? p : 0.0);
+          const double c = s.u[v][o], p = s.p[o];
+          f0 = c * V0 + p * gv[0];
+          f1 = c * V1 + p * gv[1];
+          f2 = c * V2 + p * gv[2];
         }
-        f0 = g9[0] * F + g9[1] * Gf + g9[2] * H;
-        f1 = g9[3] * F + g9[4] * Gf + g9[5] * H;
-        f2 = g9[6] * F + g9[7] * Gf + g9[8] * H;
       }
       acc = P::mfma(afr[g], f0, acc);
       acc = P::mfma(afr[G + g], f1, acc);
@@ -357,18 +401,6 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
       // keep each node group's LDS reads next to its MFMAs (bounds the
       // registers the scheduler may spend on reads hoisted from later groups)
       __builtin_amdgcn_sched_barrier(0);
-    }
-    // ---- surface rows ----
-#pragma unroll
-    for (int ks = 0; ks < KSS; ks++) {
-      const double bf = s.fs[v][(4 * ks + kk) * EB + e];
-      acc = P::mfma(afr[3 * G + ks], bf, acc);
-#pragma unroll
-      for (int r = 0; r < REM; r++) {
-        racc[r] += arw[r * KDP + KV + 4 * ks] * bf;
-        asm volatile("" : "+v"(racc[r]));
-      }
-      if (ks % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
 
     // ---- LSRK epilogue (element e, rows 16t + 4r + kk) ----
@@ -414,7 +446,7 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
         }
       }
     }
-    __syncthreads();
+    __syncthreads();  // slot free for batch b + 2*gsz
   }
 }
 
@@ -429,6 +461,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   constexpr int Np = S::Np, Nfp = S::Nfp, KDP = S::KDP, RTF = S::RTF, REM = S::REM, NC = S::NC, NT = S::NT;
 
   __shared__ WsSlot<N> sl[2];
+  __shared__ double s_fs[5 * S::NFF4 * S::EB];  // surface fluxes of the batch being consumed
   __shared__ double s_arem[REM > 0 ? REM * KDP : 1];
   __shared__ int s_fmx[16 * Nfp];
   __shared__ int s_cb[2][S::EB * 4];  // packed connectivity of the batch being staged / the next one
@@ -459,27 +492,36 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
     const int pw = w - NC, e0 = b * S::EB + S::EPW * pw;
     if (lane < S::EPW * 4 && e0 * 4 + lane < a.K * 4) s_cb[0][S::EPW * pw * 4 + lane] = a.conn[(size_t)e0 * 4 + lane];
   }
-  __syncthreads();  // s_fmx, s_cb
-  if (w >= NC && b < hi) ws_produce<N>(a, sl[0], s_fmx, s_cb[0], s_cb[1], b, b + gsz < hi ? b + gsz : -1, w - NC, lane);
-  __syncthreads();
+  __syncthreads();  // s_fmx, s_arem, s_cb
 
   if (w >= NC) {
-    // producer waves: stage the batch after the one being consumed; staging
-    // call n (n = it + 1) reads the connectivity buffer n & 1
+    // producer waves: batch b's staging precedes the loop; in the loop's
+    // iteration for batch b they stage b + gsz.  Staging call n reads the
+    // connectivity buffer n & 1 and fills the other.
+    const int pw = w - NC;
+    WsTraces<N> tr;
+    if (b < hi) ws_stage_nodes<N>(a, sl[0], s_fmx, s_cb[0], s_cb[1], b, b + gsz < hi ? b + gsz : -1, pw, lane, tr);
+    __syncthreads();
+    if (b < hi) ws_stage_surface<N>(a, sl[0], s_fs, s_fmx, b, pw, lane, tr);
+    __syncthreads();
     for (int it = 0; b < hi; b += gsz, it++) {
       const int nb = b + gsz, nnb = b + 2 * gsz;
       if (nb < hi)
-        ws_produce<N>(a, sl[(it + 1) & 1], s_fmx, s_cb[(it + 1) & 1], s_cb[it & 1], nb, nnb < hi ? nnb : -1, w - NC,
-                      lane);
+        ws_stage_nodes<N>(a, sl[(it + 1) & 1], s_fmx, s_cb[(it + 1) & 1], s_cb[it & 1], nb, nnb < hi ? nnb : -1, pw,
+                          lane, tr);
+      __syncthreads();  // consumers are done with the surface buffer
+      if (nb < hi) ws_stage_surface<N>(a, sl[(it + 1) & 1], s_fs, s_fmx, nb, pw, lane, tr);
       __syncthreads();
     }
     return;
   }
+  __syncthreads();  // batch b's nodes staged
+  __syncthreads();  // batch b's surface fluxes staged
   const int v = w / RTF, t = w - v * RTF;
   if (REM > 0 && t == 0)
-    ws_consume<N, MODE, true>(a, sl, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
+    ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
   else
-    ws_consume<N, MODE, false>(a, sl, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
+    ws_consume<N, MODE, false>(a, sl, s_fs, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
 }
 
 // --------------------------------------------------------------------------
