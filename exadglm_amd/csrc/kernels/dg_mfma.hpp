@@ -373,18 +373,35 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
       }
       __syncthreads();
       if constexpr (OFFS) {
-        // least-squares affine fit per (block, element): c_bq = sum_k fitP[k][q] B_k
-        for (int task = tid; task < NBLK * 4 * EB; task += NT) {
-          const int bb = task / (4 * EB), q = (task / EB) & 3, e = task & 15;
+        // least-squares affine fit per (block, element): c_bq = sum_k fitP[k][q] B_k.
+        // Wave w takes blocks w, w + RT, ...; lane (element e, quarter s) sums
+        // rows k0 + s, k0 + s + 4, ... of the block for all four q, and two
+        // shuffles combine the quarters (a fixed order: deterministic).
+        for (int bb = w; bb < NBLK; bb += S::RT) {
           const int k0 = bb < 3 ? bb * Np : 3 * Np + (bb - 3) * Nfp, k1 = bb < 3 ? k0 + Np : k0 + Nfp;
-          double c0 = 0.0, c1 = 0.0;
-          int k = k0;
-          for (; k + 1 < k1; k += 2) {
-            c0 += fitP[k * 4 + q] * s_b64[k * EB + e];
-            c1 += fitP[(k + 1) * 4 + q] * s_b64[(k + 1) * EB + e];
+          const int e = lane & 15, sq = lane >> 4;
+          double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+          for (int k = k0 + sq; k < k1; k += 4) {
+            const double bv = s_b64[k * EB + e];
+            const double *pk = &fitP[k * 4];
+            c0 += pk[0] * bv;
+            c1 += pk[1] * bv;
+            c2 += pk[2] * bv;
+            c3 += pk[3] * bv;
           }
-          if (k < k1) c0 += fitP[k * 4 + q] * s_b64[k * EB + e];
-          s_c[task] = c0 + c1;
+#pragma unroll
+          for (int o = 16; o <= 32; o <<= 1) {
+            c0 += __shfl_xor(c0, o, 64);
+            c1 += __shfl_xor(c1, o, 64);
+            c2 += __shfl_xor(c2, o, 64);
+            c3 += __shfl_xor(c3, o, 64);
+          }
+          if (sq == 0) {
+            s_c[(bb * 4 + 0) * EB + e] = c0;
+            s_c[(bb * 4 + 1) * EB + e] = c1;
+            s_c[(bb * 4 + 2) * EB + e] = c2;
+            s_c[(bb * 4 + 3) * EB + e] = c3;
+          }
         }
         __syncthreads();
         // the remainder B - fit, rounded to the MFMA precision

@@ -23,6 +23,7 @@
 #include "kernels/dg_mfma.hpp"
 #include "kernels/dg_ws.hpp"
 #include "kernels/dg_adjoint.hpp"
+#include "kernels/dg_adjoint_mfma.hpp"
 
 #define DGLM_NTIMERS 3  // wave speed, dt finalize, stage
 
@@ -1032,6 +1033,9 @@ void launch_adj_a(dglm_ctx *c, const AdjArgs &a) {
   hipLaunchKernelGGL((adjoint_pass_a<N, PHYS>), dim3((c->K + EB - 1) / EB), dim3(256), 0, c->stream, a);
 }
 
+template <int N>
+void launch_adj_a_mfma(dglm_ctx *c, const AdjArgs &a);
+
 template <int N, int PHYS>
 void launch_adj_b(dglm_ctx *c, const AdjArgs &a) {
   const int n = c->K * Ord<N>::Np;
@@ -1047,8 +1051,39 @@ int adj_blocks() {
   return EB;
 }
 
+// The MFMA pass A (FP64 Euler, N <= 4) where it fits a workgroup.
+template <int PHYS>
+bool pick_adj_mfma(int N, adj_fn &A, int &eb) {
+  if (PHYS != PHYS_EULER) return false;
+  switch (N) {
+#define DGLM_MCASE(n)                          \
+  case n:                                      \
+    if (!adj_mfma_available<n>()) return false; \
+    A = launch_adj_a_mfma<n>;                  \
+    eb = AdjShape<n>::EB;                      \
+    return true;
+    DGLM_MCASE(1)
+    DGLM_MCASE(2)
+    DGLM_MCASE(3)
+    DGLM_MCASE(4)
+#undef DGLM_MCASE
+    default:
+      return false;
+  }
+}
+
+template <int PHYS>
+bool pick_adj_impl(int N, adj_fn &A, adj_fn &B, lam_fn &Lf, int &eb);
+
 template <int PHYS>
 bool pick_adj(int N, adj_fn &A, adj_fn &B, lam_fn &Lf, int &eb) {
+  if (!pick_adj_impl<PHYS>(N, A, B, Lf, eb)) return false;
+  pick_adj_mfma<PHYS>(N, A, eb);
+  return true;
+}
+
+template <int PHYS>
+bool pick_adj_impl(int N, adj_fn &A, adj_fn &B, lam_fn &Lf, int &eb) {
   switch (N) {
 #define DGLM_ACASE(n)                      \
   case n:                                  \
@@ -1084,6 +1119,7 @@ StageArgs make_stage_args(dglm_ctx *c, const double *u_in, double *u_out, double
 
 struct dglm_adj_tape {
   int nsteps = 0, every = 0, nck = 0, eb = 1, nblk_a = 0;
+  double *afragT = nullptr;  // A^T fragments of the MFMA pass A (null: scalar pass A)
   double *ckpt = nullptr, *ck_t = nullptr, *dt = nullptr, *seg = nullptr, *lam = nullptr, *kscr = nullptr;
   int *fstar = nullptr, *clip = nullptr, *ck_step = nullptr;
   DevScalars *scal2 = nullptr;
@@ -1096,6 +1132,15 @@ struct dglm_adj_tape {
 namespace {
 
 template <int N>
+void launch_adj_a_mfma(dglm_ctx *c, const AdjArgs &a) {
+  if constexpr (adj_mfma_available<N>()) {
+    using S = AdjShape<N>;
+    hipLaunchKernelGGL((adjoint_pass_a_mfma<N>), dim3((c->K + S::EB - 1) / S::EB), dim3(S::NT), 0, c->stream, a,
+                       c->tape->afragT);
+  }
+}
+
+template <int N>
 void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step) {
   dglm_adj_tape *T = c->tape;
   hipLaunchKernelGGL((lambda_adjoint_kernel<N>), dim3((c->K + 255) / 256), dim3(256), 0, c->stream, c->K, u, lam,
@@ -1106,7 +1151,8 @@ void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step) {
 void free_tape(dglm_adj_tape *T) {
   if (!T) return;
   void *ptrs[] = {T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2,
-                  T->ubar, T->kbar, T->gsur, T->taubar, T->dtpart, T->tsc, T->rbar, T->slots, T->fpos};
+                  T->ubar, T->kbar, T->gsur, T->taubar, T->dtpart, T->tsc, T->rbar, T->slots, T->fpos,
+                  T->afragT};
   for (void *q : ptrs)
     if (q) hipFree(q);
   delete T;
@@ -1150,6 +1196,29 @@ int ensure_adjoint_buffers(dglm_ctx *c) {
   HIP_TRY(hipMemcpy(T->slots, slots.data(), slots.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(T->fpos, fpos.data(), fpos.size() * sizeof(int), hipMemcpyHostToDevice));
   T->bytes += (2 * nst + (size_t)c->nvar * c->K * c->NFF + (size_t)c->K * 4 + T->nblk_a) * sizeof(double);
+  // A^T fragments for the MFMA pass A
+  adj_fn Am;
+  int ebm = 0;
+  if (c->p.physics == DGLM_EULER && pick_adj_mfma<PHYS_EULER>(c->p.N, Am, ebm)) {
+    std::vector<double> opT((size_t)c->KD * Np), frag;
+    HIP_TRY(hipMemcpy(opT.data(), c->op, opT.size() * sizeof(double), hipMemcpyDeviceToHost));
+    switch (c->p.N) {
+#define DGLM_FCASE(n)                                                            \
+  case n:                                                                        \
+    frag.resize((size_t)AdjShape<n>::KT * AdjShape<n>::KSA * 64);               \
+    adj_arrange<n>(opT.data(), frag.data());                                     \
+    break;
+      DGLM_FCASE(1)
+      DGLM_FCASE(2)
+      DGLM_FCASE(3)
+      DGLM_FCASE(4)
+#undef DGLM_FCASE
+      default:
+        break;
+    }
+    if ((rc = dev_alloc(&T->afragT, frag.size(), "A^T fragments"))) return rc;
+    HIP_TRY(hipMemcpy(T->afragT, frag.data(), frag.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   return DGLM_OK;
 }
 
