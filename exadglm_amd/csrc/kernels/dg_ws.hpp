@@ -11,9 +11,10 @@
 //            (global_load_lds_dwordx4, no registers), the exterior traces of
 //            all face nodes and the wave speeds for tau are gathered into
 //            registers with the connectivity prefetched a batch earlier --
-//            then write p, the contravariant velocity V_d = sum_x g_dx v_x and
-//            tau for b+1;
-//   phase 2  consumers contract the volume rows of batch b and apply the
+//            then write tau for b+1;
+//   phase 2  consumers form p and the contravariant velocity
+//            V_d = sum_x g_dx v_x of batch b+1 (VALU work beside their MFMA
+//            chains), contract the volume rows of batch b and apply the
 //            LSRK update; producers turn b+1's traces into the LM (or
 //            Rusanov) surface flux of all five variables at every face node,
 //            written once per face node into the surface buffer that the
@@ -215,10 +216,17 @@ __device__ inline void ws_stage_nodes(const StageArgs &a, WsSlot<N> &sl, const i
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-  // pressure and contravariant velocity
-  for (int i = lane; i < EPW * Np; i += 64) {
-    const int o = el * Np + i, e = el + i / Np;
+// Pressure and contravariant velocity V_d = sum_x g_dx v_x of every node of
+// a staged batch; threads t = 0..nthr-1 (the consumer waves, whose VALU is
+// free while their MFMA chains run).
+template <int N>
+__device__ inline void ws_primitives(const StageArgs &a, WsSlot<N> &sl, int t, int nthr) {
+  using S = WsShape<N>;
+  constexpr int Np = S::Np;
+  for (int o = t; o < S::EB * Np; o += nthr) {
+    const int e = o / Np;
     const double r = sl.u[0][o], mx = sl.u[1][o], my = sl.u[2][o], mz = sl.u[3][o], E = sl.u[4][o];
     const double inv = 1.0 / r;
     sl.p[o] = (a.gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
@@ -281,7 +289,9 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
 #pragma unroll
       for (int v = 0; v < 5; v++) fs[(v * NFF4 + m) * EB + e] = fl[v];
     }
-    __builtin_amdgcn_sched_barrier(0);  // one face node at a time: bounds live registers
+    // four face nodes in flight at a time: enough independent chains to
+    // cover the FP64 latencies, few enough to bound live registers
+    if (s % 4 == 3) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -290,7 +300,7 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
 // the full tiles (a separate instantiation so both loops stay straight-line
 // code).
 template <int N, int MODE, bool WREM>
-__device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const double *fs, const double *s_arem,
+__device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const double *fs, const double *s_arem,
                                   const double *__restrict__ afrag, double dt, int b, int hi, int gsz, int v, int t,
                                   int lane) {
   using S = WsShape<N>;
@@ -356,6 +366,9 @@ __device__ inline void ws_consume(const StageArgs &a, const WsSlot<N> *sl, const
       if (ks % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();  // surface buffer free for the next batch's fluxes
+
+    // primitives of the next batch (its node values landed in phase 1)
+    if (b + gsz < hi) ws_primitives<N>(a, sl[(it + 1) & 1], (v * RTF + t) * 64 + lane, S::NC * 64);
 
     // ---- phase 2: volume rows, f_d = c V_d + p g_d,v-1 ----
     double gv[3] = {0.0, 0.0, 0.0};
@@ -516,7 +529,8 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
     return;
   }
   __syncthreads();  // batch b's nodes staged
-  __syncthreads();  // batch b's surface fluxes staged
+  if (b < hi) ws_primitives<N>(a, sl[0], tid, NC * 64);
+  __syncthreads();  // batch b's surface fluxes and primitives staged
   const int v = w / RTF, t = w - v * RTF;
   if (REM > 0 && t == 0)
     ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
