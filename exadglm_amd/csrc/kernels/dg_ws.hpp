@@ -137,9 +137,14 @@ __device__ inline void dma_copy(const double *src, double *dst, int bytes, int l
   if ((bytes & 15) && lane == 0) dst[2 * n16] = src[2 * n16];
 }
 
+// Face-node pairs (face node m, element e) are split between the producer
+// waves by face-node range, element fastest: pair P of producer pw is
+// m = pw*MPW + P/16, e = P%16, so a wave's LDS stores of one variable cover
+// whole 16-element rows of the surface buffer (no bank conflicts).
 template <int N>
 struct WsTraces {
-  static constexpr int NRS = (WsShape<N>::EPW * WsShape<N>::NFF4 + 63) / 64;
+  static constexpr int MPW = (WsShape<N>::NFF4 + WsShape<N>::NPW - 1) / WsShape<N>::NPW;  // face nodes per wave
+  static constexpr int NRS = (MPW * WsShape<N>::EB + 63) / 64;
   double up[NRS][5];  // exterior traces of this lane's face-node pairs
 };
 
@@ -182,19 +187,21 @@ __device__ inline void ws_stage_nodes(const StageArgs &a, WsSlot<N> &sl, const i
   const int ne_n = nb >= 0 ? max(0, min(EPW, K - (nb * EB + EPW * pw))) : 0;
   if (lane < ne_n * 4) cn = a.conn[(size_t)(nb * EB + EPW * pw) * 4 + lane];
 
-  // exterior traces: pair P = lane + 64 s is (element el + P / NFF4, face node P % NFF4)
+  // exterior traces of this wave's face-node pairs (see WsTraces)
+  constexpr int MPW = WsTraces<N>::MPW;
+  const int neb = max(0, min(EB, K - b * EB));  // live elements of the batch
   const double *ubase[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) ubase[v] = a.u_in + v * KNp;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int P = lane + 64 * s;
-    const int q = P / NFF4, m = P - q * NFF4;
-    const bool ok = P < EPW * NFF4 && q < ne && m < NFF;
+    const int m = pw * MPW + (P >> 4), e = P & 15;
+    const bool ok = P < MPW * EB && m < NFF && e < neb;
     int pidx = 0;  // K*Np < 2^31 (checked on the host)
     if (ok) {
       const int f = m / Nfp, jf = m - f * Nfp;
-      const int cc = cb_cur[(el + q) * 4 + f];
+      const int cc = cb_cur[e * 4 + f];
       pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
     }
 #pragma unroll
@@ -248,16 +255,15 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
   constexpr int NRS = WsTraces<N>::NRS;
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
-  const int e0 = b * EB + EPW * pw;
-  const int ne = max(0, min(EPW, a.K - e0));
-  const int el = EPW * pw;
+  constexpr int MPW = WsTraces<N>::MPW;
+  const int neb = max(0, min(EB, a.K - b * EB));  // live elements of the batch
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int P = lane + 64 * s;
-    if (P < EPW * NFF4) {
-      const int q = P / NFF4, m = P - q * NFF4, e = el + q;
+    const int m = pw * MPW + (P >> 4), e = P & 15;
+    if (P < MPW * EB && m < NFF4) {
       double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (q < ne && m < NFF) {
+      if (e < neb && m < NFF) {
         const int f = m / Nfp, jf = m - f * Nfp;
         const int o = e * Np + s_fmx[(f * 4 + 3) * Nfp + jf];
         const double *g = &sl.geo[e * GEO_STRIDE + 9 + f * 4];
