@@ -58,6 +58,9 @@ struct WsShape {
   static constexpr int NT = 64 * NW;
   static constexpr int EB = 16;               // elements per batch (MFMA columns)
   static constexpr int EPW = EB / NPW;        // elements staged per producer wave (a contiguous half)
+  static constexpr int FS = EB + 1;           // row stride of the surface buffer: a producer wave stores
+                                              // consecutive face nodes of one element, and the odd stride
+                                              // spreads them over the LDS banks
 };
 
 // One batch's node data in LDS (two of these alternate).  Node arrays are
@@ -77,7 +80,7 @@ struct WsSlot {
 template <int N>
 constexpr size_t ws_lds_bytes() {
   using S = WsShape<N>;
-  return 2 * sizeof(WsSlot<N>) + 8 * 5 * S::NFF4 * S::EB + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp +
+  return 2 * sizeof(WsSlot<N>) + 8 * 5 * S::NFF4 * S::FS + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp +
          4 * 2 * S::EB * 4;
 }
 
@@ -137,14 +140,9 @@ __device__ inline void dma_copy(const double *src, double *dst, int bytes, int l
   if ((bytes & 15) && lane == 0) dst[2 * n16] = src[2 * n16];
 }
 
-// Face-node pairs (face node m, element e) are split between the producer
-// waves by face-node range, element fastest: pair P of producer pw is
-// m = pw*MPW + P/16, e = P%16, so a wave's LDS stores of one variable cover
-// whole 16-element rows of the surface buffer (no bank conflicts).
 template <int N>
 struct WsTraces {
-  static constexpr int MPW = (WsShape<N>::NFF4 + WsShape<N>::NPW - 1) / WsShape<N>::NPW;  // face nodes per wave
-  static constexpr int NRS = (MPW * WsShape<N>::EB + 63) / 64;
+  static constexpr int NRS = (WsShape<N>::EPW * WsShape<N>::NFF4 + 63) / 64;
   double up[NRS][5];  // exterior traces of this lane's face-node pairs
 };
 
@@ -187,21 +185,19 @@ __device__ inline void ws_stage_nodes(const StageArgs &a, WsSlot<N> &sl, const i
   const int ne_n = nb >= 0 ? max(0, min(EPW, K - (nb * EB + EPW * pw))) : 0;
   if (lane < ne_n * 4) cn = a.conn[(size_t)(nb * EB + EPW * pw) * 4 + lane];
 
-  // exterior traces of this wave's face-node pairs (see WsTraces)
-  constexpr int MPW = WsTraces<N>::MPW;
-  const int neb = max(0, min(EB, K - b * EB));  // live elements of the batch
+  // exterior traces: pair P = lane + 64 s is (element el + P / NFF4, face node P % NFF4)
   const double *ubase[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) ubase[v] = a.u_in + v * KNp;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int P = lane + 64 * s;
-    const int m = pw * MPW + (P >> 4), e = P & 15;
-    const bool ok = P < MPW * EB && m < NFF && e < neb;
+    const int q = P / NFF4, m = P - q * NFF4;
+    const bool ok = P < EPW * NFF4 && q < ne && m < NFF;
     int pidx = 0;  // K*Np < 2^31 (checked on the host)
     if (ok) {
       const int f = m / Nfp, jf = m - f * Nfp;
-      const int cc = cb_cur[e * 4 + f];
+      const int cc = cb_cur[(el + q) * 4 + f];
       pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
     }
 #pragma unroll
@@ -246,7 +242,7 @@ __device__ inline void ws_primitives(const StageArgs &a, WsSlot<N> &sl, int t, i
 
 // Producer phase 2: LM (or Rusanov) surface flux of the five variables at each
 // face node of this wave's half of batch b, times Fscale, into fs
-// ([variable][face node][element]).
+// ([variable][face node][element], row stride FS).
 template <int N>
 __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl, double *fs, const int *s_fmx, int b,
                                         int pw, int lane_in, const WsTraces<N> &tr) {
@@ -255,15 +251,16 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
   constexpr int NRS = WsTraces<N>::NRS;
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
-  constexpr int MPW = WsTraces<N>::MPW;
-  const int neb = max(0, min(EB, a.K - b * EB));  // live elements of the batch
+  const int e0 = b * EB + EPW * pw;
+  const int ne = max(0, min(EPW, a.K - e0));
+  const int el = EPW * pw;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int P = lane + 64 * s;
-    const int m = pw * MPW + (P >> 4), e = P & 15;
-    if (P < MPW * EB && m < NFF4) {
+    if (P < EPW * NFF4) {
+      const int q = P / NFF4, m = P - q * NFF4, e = el + q;
       double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (e < neb && m < NFF) {
+      if (q < ne && m < NFF) {
         const int f = m / Nfp, jf = m - f * Nfp;
         const int o = e * Np + s_fmx[(f * 4 + 3) * Nfp + jf];
         const double *g = &sl.geo[e * GEO_STRIDE + 9 + f * 4];
@@ -293,7 +290,7 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
         }
       }
 #pragma unroll
-      for (int v = 0; v < 5; v++) fs[(v * NFF4 + m) * EB + e] = fl[v];
+      for (int v = 0; v < 5; v++) fs[(v * NFF4 + m) * S::FS + e] = fl[v];
     }
     // four face nodes in flight at a time: enough independent chains to
     // cover the FP64 latencies, few enough to bound live registers
@@ -328,7 +325,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
     afr[ks] = afrag[((size_t)t * KS + ks) * 64 + lane];
     asm volatile("" : "+v"(afr[ks]));
   }
-  const double *fsv = fs + v * NFF4 * EB;
+  const double *fsv = fs + v * NFF4 * S::FS;
 
   for (int it = 0; b < hi; b += gsz, it++) {
     const WsSlot<N> &s = sl[it & 1];
@@ -362,7 +359,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
     // ---- phase 1: surface rows ----
 #pragma unroll
     for (int ks = 0; ks < KSS; ks++) {
-      const double bf = fsv[(4 * ks + kk) * EB + e];
+      const double bf = fsv[(4 * ks + kk) * S::FS + e];
       acc = P::mfma(afr[3 * G + ks], bf, acc);
 #pragma unroll
       for (int r = 0; r < REM; r++) {
@@ -480,7 +477,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   constexpr int Np = S::Np, Nfp = S::Nfp, KDP = S::KDP, RTF = S::RTF, REM = S::REM, NC = S::NC, NT = S::NT;
 
   __shared__ WsSlot<N> sl[2];
-  __shared__ double s_fs[5 * S::NFF4 * S::EB];  // surface fluxes of the batch being consumed
+  __shared__ double s_fs[5 * S::NFF4 * S::FS];  // surface fluxes of the batch being consumed
   __shared__ double s_arem[REM > 0 ? REM * KDP : 1];
   __shared__ int s_fmx[16 * Nfp];
   __shared__ int s_cb[2][S::EB * 4];  // packed connectivity of the batch being staged / the next one
