@@ -269,30 +269,51 @@ __global__ __launch_bounds__(256) void adjoint_pass_b(AdjArgs a) {
   const int e = idx / Np, i = idx - e * Np;
   double w[NV];
   for (int v = 0; v < NV; v++) w[v] = 0.0;
+  // all loads of the (at most 3) partner slots first, then the arithmetic
+  int ms[3];
+  size_t pidx[3];
+  const double *gn[3], *gs[3];
+  double tau_n[3];
+#pragma unroll
   for (int q = 0; q < 3; q++) {
-    const int mslot = a.node_slots[i * 3 + q];
-    if (mslot < 0) break;
-    const int f = mslot / Nfp, j = mslot - f * Nfp;
-    const int c = a.conn[(size_t)e * 4 + f];
-    const int nb = conn_nbr(c), nf = conn_face(c);
-    const int pnode = a.fmx[(nf * 4 + conn_code(c)) * Nfp + j];  // neighbour's volume node
-    const int jp = a.fpos[nf * Np + pnode];                       // its slot on face nf
-    const size_t pidx = (size_t)nb * Np + pnode;
-    const double *gn = &a.geo[(size_t)nb * GEO_STRIDE + 9 + nf * 4];
-    const double tau_n = 0.25 * (a.maxv[(size_t)nb * 4 + nf] + a.maxv[(size_t)e * 4 + f]) * a.tau_scale;
-    const double *gs = &a.gsur[((size_t)nb * NFF + nf * Nfp + jp) * NV];
-    if (PHYS == PHYS_EULER) {
+    ms[q] = a.node_slots[i * 3 + q];
+    if (ms[q] >= 0) {
+      const int f = ms[q] / Nfp, j = ms[q] - f * Nfp;
+      const int c = a.conn[(size_t)e * 4 + f];
+      const int nb = conn_nbr(c), nf = conn_face(c);
+      const int pnode = a.fmx[(nf * 4 + conn_code(c)) * Nfp + j];  // neighbour's volume node
+      const int jp = a.fpos[nf * Np + pnode];                       // its slot on face nf
+      pidx[q] = (size_t)nb * Np + pnode;
+      gn[q] = &a.geo[(size_t)nb * GEO_STRIDE + 9 + nf * 4];
+      tau_n[q] = 0.25 * (a.maxv[(size_t)nb * 4 + nf] + a.maxv[(size_t)e * 4 + f]) * a.tau_scale;
+      gs[q] = &a.gsur[((size_t)nb * NFF + nf * Nfp + jp) * NV];
+    }
+  }
+  if (PHYS == PHYS_EULER) {
+    double uo[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) uo[v] = a.u[v * KNp + idx];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (ms[q] < 0) break;
       double ub[5], h[5];
+#pragma unroll
       for (int v = 0; v < 5; v++) {
-        ub[v] = 0.5 * (a.u[v * KNp + pidx] + a.u[v * KNp + idx]);
-        h[v] = 0.5 * gn[3] * gs[v];
+        ub[v] = 0.5 * (a.u[v * KNp + pidx[q]] + uo[v]);
+        h[v] = 0.5 * gn[q][3] * gs[q][v];
       }
       double ww[5] = {0, 0, 0, 0, 0};
-      euler_jt(ub, gn[0], gn[1], gn[2], h, a.gamma, ww);
-      for (int v = 0; v < 5; v++) w[v] += ww[v] - tau_n * h[v];
-    } else {
-      const double vn = gn[0] * a.vel[pidx] + gn[1] * a.vel[KNp + pidx] + gn[2] * a.vel[2 * KNp + pidx];
-      w[0] += gn[3] * (0.5 * vn - 0.5 * tau_n) * gs[0];
+      euler_jt(ub, gn[q][0], gn[q][1], gn[q][2], h, a.gamma, ww);
+#pragma unroll
+      for (int v = 0; v < 5; v++) w[v] += ww[v] - tau_n[q] * h[v];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      if (ms[q] < 0) break;
+      const size_t pi = pidx[q];
+      const double vn = gn[q][0] * a.vel[pi] + gn[q][1] * a.vel[KNp + pi] + gn[q][2] * a.vel[2 * KNp + pi];
+      w[0] += gn[q][3] * (0.5 * vn - 0.5 * tau_n[q]) * gs[q][0];
     }
   }
   for (int v = 0; v < NV; v++) a.ubar[v * KNp + idx] += w[v];
@@ -328,66 +349,95 @@ __global__ __launch_bounds__(256) void adjoint_dt_reduce(const double *__restric
 // of face f and, for the face attaining max(lambda*Fscale), dt.  The
 // derivative of lambda = sqrt(max v^2) + sqrt(max |gamma p/rho|) follows the
 // argmax nodes the forward picked (first strict maximum, as calc_maxv).
-// One thread per element (deterministic accumulation over its faces).
+// One thread per face side (64 elements x 4 faces per block) finds the argmax
+// nodes and forms their ubar increments into LDS; one thread per element then
+// adds them in face order, so the accumulation order per node is fixed and
+// the result deterministic.
+constexpr int LAM_ADJ_EB = 64;
+
 template <int N>
-__global__ __launch_bounds__(256) void lambda_adjoint_kernel(int K, const double *__restrict__ u,
-                                                             const double *__restrict__ lam,
-                                                             const double *__restrict__ taubar,
-                                                             const int *__restrict__ conn,
-                                                             const int *__restrict__ fmx, const double *dt_seq,
-                                                             const int *fstar_seq, int step, const double *scal,
-                                                             double tau_scale, double gamma, double *ubar) {
+__global__ __launch_bounds__(4 * LAM_ADJ_EB) void lambda_adjoint_kernel(
+    int K, const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ taubar,
+    const int *__restrict__ conn, const int *__restrict__ fmx, const double *dt_seq, const int *fstar_seq, int step,
+    const double *scal, double tau_scale, double gamma, double *ubar) {
   using O = Ord<N>;
   constexpr int Np = O::Np, Nfp = O::Nfp;
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= K) return;
+  constexpr int EB = LAM_ADJ_EB;
+  __shared__ int s_nv[4 * EB], s_nc[4 * EB];  // argmax volume nodes (-1: none)
+  __shared__ double s_dv[4][4 * EB];          // increments of rho, rho v at the |v|^2 argmax
+  __shared__ double s_dc[5][4 * EB];          // increments of all five at the c^2 argmax
   const size_t KNp = (size_t)K * Np;
-  const double dtbar = scal[1], dt = dt_seq[step];
-  const int fstar = fstar_seq[step];
-  for (int f = 0; f < 4; f++) {
+  const int t = threadIdx.x;
+  const int e = blockIdx.x * EB + (t >> 2), f = t & 3;
+  int nv = -1, nc = -1;
+  if (e < K) {
+    const double dtbar = scal[1], dt = dt_seq[step];
     const int o = e * 4 + f;
     const int c = conn[o];
     const int nbf = conn_nbr(c) * 4 + conn_face(c);
     double lb = 0.25 * tau_scale * (taubar[o] + taubar[nbf]);
-    if (o == fstar) lb += -dtbar * dt / lam[o];
-    if (lb == 0.0) continue;
-    const int *fm = fmx + (f * 4 + 3) * Nfp;
-    double mv2 = 0.0, mc2 = 0.0;
-    int jv = -1, jc = -1;
-    for (int j = 0; j < Nfp; j++) {
-      const size_t idx = (size_t)e * Np + fm[j];
-      const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx], u5 = u[4 * KNp + idx];
-      const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
-      const double v2 = vx * vx + vy * vy + vz * vz;
-      const double c2 = fabs(gamma * ((gamma - 1.0) * (u5 - 0.5 * u1 * v2)) / u1);
-      if (v2 > mv2) {
-        mv2 = v2;
-        jv = j;
+    if (o == fstar_seq[step]) lb += -dtbar * dt / lam[o];
+    if (lb != 0.0) {
+      const int *fm = fmx + (f * 4 + 3) * Nfp;
+      double mv2 = 0.0, mc2 = 0.0;
+      int jv = -1, jc = -1;
+#pragma unroll 3
+      for (int j = 0; j < Nfp; j++) {
+        const size_t idx = (size_t)e * Np + fm[j];
+        const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx],
+                     u5 = u[4 * KNp + idx];
+        const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        const double c2 = fabs(gamma * ((gamma - 1.0) * (u5 - 0.5 * u1 * v2)) / u1);
+        if (v2 > mv2) {
+          mv2 = v2;
+          jv = j;
+        }
+        if (c2 > mc2) {
+          mc2 = c2;
+          jc = j;
+        }
       }
-      if (c2 > mc2) {
-        mc2 = c2;
-        jc = j;
+      if (jv >= 0) {
+        nv = fm[jv];
+        const size_t idx = (size_t)e * Np + nv;
+        const double r = u[idx], v[3] = {u[KNp + idx] / r, u[2 * KNp + idx] / r, u[3 * KNp + idx] / r};
+        const double s = lb / (2.0 * sqrt(mv2));
+        s_dv[0][t] = s * (-2.0 * mv2 / r);
+        for (int q = 0; q < 3; q++) s_dv[1 + q][t] = s * (2.0 * v[q] / r);
+      }
+      if (jc >= 0) {
+        nc = fm[jc];
+        const size_t idx = (size_t)e * Np + nc;
+        const double r = u[idx], v[3] = {u[KNp + idx] / r, u[2 * KNp + idx] / r, u[3 * KNp + idx] / r};
+        const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        const double gm = gamma - 1.0;
+        const double p = gm * (u[4 * KNp + idx] - 0.5 * r * v2);
+        const double sg = gamma * p / r >= 0.0 ? 1.0 : -1.0;
+        const double s = sg * lb / (2.0 * sqrt(mc2));
+        s_dc[0][t] = s * gamma * (gm * 0.5 * v2 / r - p / (r * r));
+        for (int q = 0; q < 3; q++) s_dc[1 + q][t] = s * gamma * (-gm * v[q]) / r;
+        s_dc[4][t] = s * gamma * gm / r;
       }
     }
-    if (jv >= 0) {
-      const size_t idx = (size_t)e * Np + fm[jv];
-      const double r = u[idx], v[3] = {u[KNp + idx] / r, u[2 * KNp + idx] / r, u[3 * KNp + idx] / r};
-      const double s = lb / (2.0 * sqrt(mv2));
-      ubar[idx] += s * (-2.0 * mv2 / r);
-      for (int q = 0; q < 3; q++) ubar[(1 + q) * KNp + idx] += s * (2.0 * v[q] / r);
-    }
-    if (jc >= 0) {
-      const size_t idx = (size_t)e * Np + fm[jc];
-      const double r = u[idx], v[3] = {u[KNp + idx] / r, u[2 * KNp + idx] / r, u[3 * KNp + idx] / r};
-      const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-      const double gm = gamma - 1.0;
-      const double p = gm * (u[4 * KNp + idx] - 0.5 * r * v2);
-      const double sg = gamma * p / r >= 0.0 ? 1.0 : -1.0;
-      const double s = sg * lb / (2.0 * sqrt(mc2));
-      ubar[idx] += s * gamma * (gm * 0.5 * v2 / r - p / (r * r));
-      for (int q = 0; q < 3; q++) ubar[(1 + q) * KNp + idx] += s * gamma * (-gm * v[q]) / r;
-      ubar[4 * KNp + idx] += s * gamma * gm / r;
-    }
+  }
+  s_nv[t] = nv;
+  s_nc[t] = nc;
+  __syncthreads();
+  if (t < EB) {
+    const int el = blockIdx.x * EB + t;
+    if (el < K)
+      for (int ff = 0; ff < 4; ff++) {
+        const int q = 4 * t + ff;
+        if (s_nv[q] >= 0) {
+          const size_t idx = (size_t)el * Np + s_nv[q];
+          for (int v = 0; v < 4; v++) ubar[v * KNp + idx] += s_dv[v][q];
+        }
+        if (s_nc[q] >= 0) {
+          const size_t idx = (size_t)el * Np + s_nc[q];
+          for (int v = 0; v < 5; v++) ubar[v * KNp + idx] += s_dc[v][q];
+        }
+      }
   }
 }
 

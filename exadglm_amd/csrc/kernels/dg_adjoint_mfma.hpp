@@ -1,19 +1,23 @@
 // Pass A of the discrete Euler adjoint with the contraction g = A^T rbar on
-// FP64 MFMA (gfx950).  Same mathematics as adjoint_pass_a (dg_adjoint.hpp);
-// the difference is the layout of the work:
+// FP64 MFMA (gfx950).  Same mathematics as adjoint_pass_a (dg_adjoint.hpp),
+// except that the dt partial sum_nodes kb . r is formed from the quantities
+// of phase 2 (homogeneity of the Euler flux) rather than from differences of
+// the stored stage states; the other difference is the layout of the work:
 //
 //   * a workgroup takes 16 elements (the MFMA columns) and has one wave per
 //     16-row tile of A^T (KT = ceil(KD/16) waves, 11 at N=4); wave T keeps
 //     the A^T fragments of its rows for the k-steps over the Np nodes in
 //     registers and runs ceil(Np/4) v_mfma_f64_16x16x4f64 per variable;
-//   * phase 0 stages u, forms rbar = dt (kbar + b ubar) (and updates kbar,
-//     the dt partial sum) into an LDS tile [node][element];
+//   * each thread owns one volume node of the 16 elements; phase 0 loads its
+//     u, ubar and kbar into registers, updates kbar and puts
+//     rbar = dt (kbar + b ubar) into an LDS tile [node][element];
 //   * phase 1 is the contraction, g[v][k][e] into LDS (all five variables,
 //     since the transposed flux Jacobians mix them);
 //   * phase 2 is pointwise per volume node: the transposed volume Jacobians,
-//     the node's own-face terms (exterior trace gathered from global memory)
-//     and the write of ubar; then tau-bar per face side and the face part of
-//     g for pass B.
+//     the node's own-face terms (exterior traces gathered from global memory
+//     ahead of the volume terms) and the write of ubar; each face-node slot
+//     leaves its tau-bar partial in LDS, summed per face side afterwards; then
+//     the face part of g for pass B.
 #pragma once
 #include "dg_adjoint.hpp"
 #include "dg_mfma.hpp"
@@ -34,7 +38,7 @@ struct AdjShape {
 template <int N>
 constexpr size_t adj_mfma_lds_bytes() {
   using S = AdjShape<N>;
-  return 8 * (5 * S::EB * S::Np + 5 * S::NpA * S::EB + 5 * S::KD * S::EB + S::EB * GEO_STRIDE + S::EB * 4 +
+  return 8 * (5 * S::NpA * S::EB + 5 * S::KD * S::EB + S::EB * GEO_STRIDE + S::EB * 4 +
               S::NT / 64) +
          4 * (S::EB * 4 + 16 * S::Nfp + S::Np * 3);
 }
@@ -64,7 +68,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   using P = PrecTraits<PREC_F64>;
   constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, KD = S::KD, KSA = S::KSA, NpA = S::NpA, NT = S::NT;
   constexpr int EB = S::EB;
-  __shared__ double s_u[5][EB * Np];   // [element][node]
+  static_assert(EB * Np <= NT, "one volume node per thread");
   __shared__ double s_r[5][NpA * EB];  // rbar, [node][element]
   __shared__ double s_g[5][KD * EB];   // A^T rbar, [row][element]
   __shared__ double s_geo[EB * GEO_STRIDE];
@@ -73,6 +77,10 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   __shared__ int s_fmx[16 * Nfp];
   __shared__ int s_slots[Np * 3];
   __shared__ double red[NT / 64];
+  // tau-bar partial of each face-node slot, [element][slot]; rbar is dead
+  // once the contraction is done
+  static_assert(EB * NFF <= 5 * NpA * EB, "slot partials must fit the rbar tile");
+  double *const s_tc = &s_r[0][0];
 
   const int K = a.K;
   const size_t KNp = (size_t)K * Np;
@@ -81,8 +89,13 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // thread tid owns volume node j of element e of the block for the whole
+  // kernel: its state and ubar stay in registers from phase 0 to phase 2
+  const bool live = tid < ne * Np;
+  const int e = tid / Np, j = tid - e * Np;
+  const size_t g0 = (size_t)e0 * Np + tid;
 
-  // ---- phase 0: staging, rbar, kbar update, dt partial ----
+  // ---- phase 0: staging, rbar, kbar update ----
   for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
   for (int i = tid; i < Np * 3; i += NT) s_slots[i] = a.node_slots[i];
   for (int i = tid; i < ne * GEO_STRIDE; i += NT) s_geo[i] = a.geo[(size_t)e0 * GEO_STRIDE + i];
@@ -91,47 +104,38 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
     s_conn[i] = c;
     s_tau[i] = 0.25 * (a.maxv[(size_t)e0 * 4 + i] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) * a.tau_scale;
   }
-  const double dt = a.mode == 0 ? *a.dtp : 0.0;
-  double dtsum = 0.0;
   for (int v = 0; v < 5; v++) {
     for (int i = tid; i < NpA * EB; i += NT) s_r[v][i] = 0.0;  // padded rows and columns
   }
-  __syncthreads();
+  const double dt = a.mode == 0 ? *a.dtp : 0.0;
+  double U[5], UB[5], R[5];
+  if (live) {
+    // every load before any store (kbar may alias nothing, but the compiler
+    // cannot know): one round trip instead of five
 #pragma unroll
-  for (int v = 0; v < 5; v++)
-    for (int i = tid; i < ne * Np; i += NT) {
-      const size_t gi = v * KNp + (size_t)e0 * Np + i;
-      const int e = i / Np, j = i - e * Np;
-      const double uu = a.u[gi];
-      s_u[v][i] = uu;
-      double r;
+    for (int v = 0; v < 5; v++) {
+      U[v] = a.u[v * KNp + g0];
       if (a.mode == 0) {
-        const double ub = a.ubar[gi];
-        const double kb = a.kbar[gi] + a.b * ub;
-        a.kbar[gi] = a.a * kb;
-        r = dt * kb;
-        if (a.dtpart && dt != 0.0) {
-          const double knext = (a.u_next[gi] - uu) / a.b;
-          const double kcur = a.u_prev ? (uu - a.u_prev[gi]) / a.b_prev : 0.0;
-          dtsum += (knext - a.a * kcur) / dt * kb;
-        }
+        UB[v] = a.ubar[v * KNp + g0];
+        R[v] = a.kbar[v * KNp + g0];
       } else {
-        r = a.rbar[gi];
+        UB[v] = 0.0;
+        R[v] = a.rbar[v * KNp + g0];
       }
-      s_r[v][j * EB + e] = r;
     }
-  if (a.mode == 0 && a.dtpart) {
-    // deterministic block sum into this block's slot: a fixed shuffle tree in
-    // each wave, then the wave sums in order
+    if (a.mode == 0) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) dtsum += __shfl_xor(dtsum, o, 64);
-    if (lane == 0) red[w] = dtsum;
-    __syncthreads();
-    if (tid == 0) {
-      double t = 0.0;
-      for (int q = 0; q < NT / 64; q++) t += red[q];
-      a.dtpart[blockIdx.x] += t;
+      for (int v = 0; v < 5; v++) {
+        const double kb = R[v] + a.b * UB[v];
+        a.kbar[v * KNp + g0] = a.a * kb;
+        R[v] = dt * kb;
+      }
     }
+  }
+  __syncthreads();  // s_r zeroed
+  if (live) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) s_r[v][j * EB + e] = R[v];
   }
   __syncthreads();
 
@@ -140,28 +144,46 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
     double afr[KSA];
 #pragma unroll
     for (int ks = 0; ks < KSA; ks++) afr[ks] = afragT[((size_t)w * KSA + ks) * 64 + lane];
-    const int kk = lane >> 4, e = lane & 15;
+    const int kk = lane >> 4, ec = lane & 15;
 #pragma unroll 1
     for (int v = 0; v < 5; v++) {
       typename P::Acc acc = {0, 0, 0, 0};
 #pragma unroll
-      for (int ks = 0; ks < KSA; ks++) acc = P::mfma(afr[ks], s_r[v][(4 * ks + kk) * EB + e], acc);
+      for (int ks = 0; ks < KSA; ks++) acc = P::mfma(afr[ks], s_r[v][(4 * ks + kk) * EB + ec], acc);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int k = 16 * w + 4 * r + kk;
-        if (k < KD) s_g[v][k * EB + e] = acc[r];
+        if (k < KD) s_g[v][k * EB + ec] = acc[r];
       }
     }
   }
   __syncthreads();
 
-  // ---- phase 2: transposed Jacobians per volume node ----
-  for (int idx = tid; idx < ne * Np; idx += NT) {
-    const int e = idx / Np, j = idx - e * Np;
+  // ---- phase 2: transposed Jacobians at the thread's volume node ----
+  // With the dt partial: sum over rows of g . B(u) = sum_nodes rbar . r, and
+  // B is known from what is formed here because the Euler flux is
+  // homogeneous of degree one, F(U) = J(U) U: the volume part is
+  // sum_x h^x . F^x(U) = (sum_x J_x^T h^x) . U, the face part of slot m is
+  // g_m . (n.F(ub) - tau (ub - u-)) Fscale = 2 (J_n(ub)^T h_m) . ub + tau tc_m.
+  double rdot = 0.0;
+  if (live) {
     const double *m = &s_geo[e * GEO_STRIDE];
-    double U[5], wv[5] = {0, 0, 0, 0, 0};
+    // exterior traces of the (at most 3) faces through this node, gathered
+    // first so that their latency overlaps the volume terms
+    int ms[3];
+    double uP[3][5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) U[v] = s_u[v][idx];
+    for (int q = 0; q < 3; q++) {
+      ms[q] = s_slots[j * 3 + q];
+      if (ms[q] >= 0) {
+        const int f = ms[q] / Nfp, jj = ms[q] - f * Nfp;
+        const int c = s_conn[e * 4 + f];
+        const size_t pidx = (size_t)conn_nbr(c) * Np + s_fmx[(conn_face(c) * 4 + conn_code(c)) * Nfp + jj];
+#pragma unroll
+        for (int v = 0; v < 5; v++) uP[q][v] = a.u[v * KNp + pidx];
+      }
+    }
+    double wv[5] = {0, 0, 0, 0, 0};
     // volume: sum_x J_x(u_j)^T h^x,  h^x_v = sum_d metric[d][x] g[d*Np+j][v]
 #pragma unroll
     for (int x = 0; x < 3; x++) {
@@ -173,57 +195,68 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
       euler_jt(U, x == 0 ? 1.0 : 0.0, x == 1 ? 1.0 : 0.0, x == 2 ? 1.0 : 0.0, h, a.gamma, wv);
       __builtin_amdgcn_sched_barrier(0);  // one direction at a time: bounds live registers
     }
-    // own faces through this node (at most 3 slots)
-#pragma unroll 1
+#pragma unroll
+    for (int v = 0; v < 5; v++) rdot += wv[v] * U[v];
+    // own faces through this node; the face-node slot's tau-bar partial
+    // tc = sum_v g[m][v] * (-(ub_v - u-_v) Fscale) goes to LDS
+#pragma unroll
     for (int q = 0; q < 3; q++) {
-      const int mslot = s_slots[j * 3 + q];
-      if (mslot < 0) break;
-      const int f = mslot / Nfp, jj = mslot - f * Nfp;
-      const int c = s_conn[e * 4 + f];
-      const size_t pidx = (size_t)conn_nbr(c) * Np + s_fmx[(conn_face(c) * 4 + conn_code(c)) * Nfp + jj];
+      if (ms[q] < 0) break;
+      const int mslot = ms[q];
+      const int f = mslot / Nfp;
       const double *g = &s_geo[e * GEO_STRIDE + 9 + f * 4];
       const double tau = s_tau[e * 4 + f];
       double ub[5], h[5], ww[5] = {0, 0, 0, 0, 0};
+      double tc = 0.0;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        ub[v] = 0.5 * (U[v] + a.u[v * KNp + pidx]);
-        h[v] = 0.5 * g[3] * s_g[v][(3 * Np + mslot) * EB + e];
+        const double gm = s_g[v][(3 * Np + mslot) * EB + e];
+        ub[v] = 0.5 * (U[v] + uP[q][v]);
+        h[v] = 0.5 * g[3] * gm;
+        tc += gm * (-(ub[v] - U[v]) * g[3]);
       }
+      s_tc[e * NFF + mslot] = tc;
       euler_jt(ub, g[0], g[1], g[2], h, a.gamma, ww);
+      double fw = 0.0;
 #pragma unroll
-      for (int v = 0; v < 5; v++) wv[v] += ww[v] + tau * h[v];
+      for (int v = 0; v < 5; v++) {
+        wv[v] += ww[v] + tau * h[v];
+        fw += ww[v] * ub[v];
+      }
+      rdot += 2.0 * fw + tau * tc;
+      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-      const size_t gi = v * KNp + (size_t)e0 * Np + idx;
-      a.ubar[gi] = (a.mode == 0 ? a.ubar[gi] : 0.0) + wv[v];
-    }
+    for (int v = 0; v < 5; v++) a.ubar[v * KNp + g0] = UB[v] + wv[v];
   }
-  // tau adjoint per face side: sum_m sum_v g[m][v] * (-(ub_v - u-_v) Fscale)
+  if (a.mode == 0 && a.dtpart && dt != 0.0) {
+    // deterministic block sum of rdot/dt = sum_nodes kb . r into this block's
+    // slot: a fixed shuffle tree in each wave, then the wave sums in order
+    double dtsum = rdot / dt;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dtsum += __shfl_xor(dtsum, o, 64);
+    if (lane == 0) red[w] = dtsum;
+  }
+  // tau adjoint per face side: the slot partials in face-node order
+  if (a.taubar || (a.mode == 0 && a.dtpart && dt != 0.0)) __syncthreads();
+  if (a.mode == 0 && a.dtpart && dt != 0.0 && tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < NT / 64; q++) t += red[q];
+    a.dtpart[blockIdx.x] += t;
+  }
   if (a.taubar)
     for (int idx = tid; idx < ne * 4; idx += NT) {
-      const int e = idx >> 2, f = idx & 3;
-      const int c = s_conn[idx];
-      const double fs = s_geo[e * GEO_STRIDE + 9 + f * 4 + 3];
+      const int el = idx >> 2, f = idx & 3;
       double t = 0.0;
-#pragma unroll 1
-      for (int j = 0; j < Nfp; j++) {
-        const int mloc = e * Np + s_fmx[(f * 4 + 3) * Nfp + j];
-        const size_t pidx = (size_t)conn_nbr(c) * Np + s_fmx[(conn_face(c) * 4 + conn_code(c)) * Nfp + j];
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
-          const double um = s_u[v][mloc];
-          const double ub = 0.5 * (um + a.u[v * KNp + pidx]);
-          t += s_g[v][(3 * Np + f * Nfp + j) * EB + e] * (-(ub - um) * fs);
-        }
-      }
+      for (int jj = 0; jj < Nfp; jj++) t += s_tc[el * NFF + f * Nfp + jj];
       const size_t o = (size_t)e0 * 4 + idx;
       a.taubar[o] = (a.mode == 0 ? a.taubar[o] : 0.0) + t;
     }
   // face part of g for pass B, [K][NFF][5]
   for (int idx = tid; idx < ne * NFF * 5; idx += NT) {
-    const int e = idx / (NFF * 5), r = idx - e * NFF * 5, mm = r / 5, v = r - mm * 5;
-    a.gsur[(size_t)(e0 + e) * NFF * 5 + r] = s_g[v][(3 * Np + mm) * EB + e];
+    const int el = idx / (NFF * 5), r = idx - el * NFF * 5, mm = r / 5, v = r - mm * 5;
+    a.gsur[(size_t)(e0 + el) * NFF * 5 + r] = s_g[v][(3 * Np + mm) * EB + el];
   }
 }
 
