@@ -53,11 +53,13 @@ struct WsShape {
   static constexpr int RTF = Np / 16;         // full MFMA row tiles
   static constexpr int REM = Np - 16 * RTF;   // rows accumulated on the VALU
   static constexpr int NC = 5 * RTF;          // consumer waves
-  static constexpr int NPW = 2;               // producer waves
+  static constexpr int NPW = 6;               // producer waves
   static constexpr int NW = NC + NPW;
   static constexpr int NT = 64 * NW;
   static constexpr int EB = 16;               // elements per batch (MFMA columns)
-  static constexpr int EPW = EB / NPW;        // elements staged per producer wave (a contiguous half)
+  static constexpr int EH = EB / 2;           // elements per DMA chunk (a half batch)
+  static constexpr int NPAIR = EB * NFF4;     // face-node pairs of a batch
+  static constexpr int PPW = (NPAIR + NPW - 1) / NPW;  // pairs per producer wave
   static constexpr int FS = EB + 1;           // row stride of the surface buffer: a producer wave stores
                                               // consecutive face nodes of one element, and the odd stride
                                               // spreads them over the LDS banks
@@ -80,8 +82,8 @@ struct WsSlot {
 template <int N>
 constexpr size_t ws_lds_bytes() {
   using S = WsShape<N>;
-  return 2 * sizeof(WsSlot<N>) + 8 * 5 * S::NFF4 * S::FS + 8 * S::REM * S::KDP + 4 * 16 * S::Nfp +
-         4 * 2 * S::EB * 4;
+  return 2 * sizeof(WsSlot<N>) + 8 * 5 * S::NFF4 * S::FS + 8 * S::REM * S::KDP + 8 * S::RTF * S::KSS * 64 +
+         4 * 16 * S::Nfp + 4 * 2 * S::EB * 4;
 }
 
 template <int N>
@@ -140,24 +142,57 @@ __device__ inline void dma_copy(const double *src, double *dst, int bytes, int l
   if ((bytes & 15) && lane == 0) dst[2 * n16] = src[2 * n16];
 }
 
+// Role of hardware wave hw of the workgroup: r < NC is consumer (v, t) =
+// (r / RTF, r % RTF), r >= NC producer r - NC.  Waves are dealt to the four
+// SIMDs round-robin (SIMD hw % 4).  At N = 4 (16 waves: five consumers with
+// VALU remainder rows, five without, six producers) the table puts three
+// consumers and one producer on SIMDs 0 and 1 and two of each on SIMDs 2 and
+// 3, with the remainder-row consumers spread 1, 1, 2, 1, which evens out the
+// FP64 work (MFMA and VALU share it) per SIMD; otherwise roles follow the
+// wave index.
 template <int N>
-struct WsTraces {
-  static constexpr int NRS = (WsShape<N>::EPW * WsShape<N>::NFF4 + 63) / 64;
-  double up[NRS][5];  // exterior traces of this lane's face-node pairs
+__device__ inline int ws_role(int hw) {
+  using S = WsShape<N>;
+  if constexpr (S::NW == 16 && S::RTF == 2 && S::REM > 0) {
+    constexpr int tab[16] = {0, 2, 4, 8, 1, 5, 6, 9, 3, 7, 12, 14, 10, 11, 13, 15};
+    int r = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) r = i == hw ? tab[i] : r;
+    return r;
+  } else {
+    return hw;
+  }
+}
+
+// Registers a producer wave carries from one staging step to the next.
+template <int N>
+struct WsProd {
+  static constexpr int NRS = (WsShape<N>::PPW + 63) / 64;
+  double up[NRS][5];        // exterior traces of this lane's face-node pairs
+  double lam_own, lam_nbr;  // wave speeds of face `lane` of the batch (producer 0)
+  int cn;                   // packed connectivity of face `lane` of the batch after (producer 0)
 };
 
-// Producer phase 1 for batch b (elements [8pw, 8pw+8) of the batch): issue
-// every load, then write tau, p and V into slot sl.  cb_cur holds the batch's
-// packed connectivity; that of batch nb (< 0: none) is loaded into cb_next.
-// The exterior traces stay in registers (tr) for phase 2.  Every LDS entry the
-// wave reads back it wrote itself, so a wave-level fence is the only
-// synchronisation inside the producer.
+// Producer staging of batch b, spread over two batch periods:
+//   ws_gather  (phase 2, a period ahead) register loads: the exterior trace
+//              of face-node pairs P = PPW*pw + lane + 64 s of the batch
+//              (element P / NFF4, face node P % NFF4, connectivity from
+//              cb_cur), and for producer 0 the wave speeds of the batch's 64
+//              faces and the connectivity of batch nb (< 0: none);
+//   ws_dma     (phase 1) node values and geometry records straight into the
+//              LDS slot (global_load_lds_dwordx4, no registers), two of the
+//              twelve half-batch copies per producer, and the padding columns
+//              of a short batch;
+//   ws_land    (phase 1) wait for all of it; producer 0 writes tau and the
+//              next connectivity.
+// The other producers, and the consumers, read what one producer wrote only
+// after the barrier that ends the phase.
 template <int N>
-__device__ inline void ws_stage_nodes(const StageArgs &a, WsSlot<N> &sl, const int *s_fmx, const int *cb_cur,
-                                      int *cb_next, int b, int nb, int pw, int lane_in, WsTraces<N> &tr) {
+__device__ inline void ws_gather(const StageArgs &a, const int *s_fmx, const int *cb_cur, int b, int nb, int pw,
+                                 int lane_in, WsProd<N> &pr) {
   using S = WsShape<N>;
-  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, EPW = S::EPW, NFF4 = S::NFF4;
-  constexpr int NRS = WsTraces<N>::NRS;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4, PPW = S::PPW;
+  constexpr int NRS = WsProd<N>::NRS;
   // lane-derived indices are the same every call: an opaque copy of the lane
   // id keeps them recomputed here rather than held in registers across the
   // batch loop
@@ -165,65 +200,77 @@ __device__ inline void ws_stage_nodes(const StageArgs &a, WsSlot<N> &sl, const i
   asm volatile("" : "+v"(lane));
   const int K = a.K;
   const size_t KNp = (size_t)K * Np;
-  const int e0 = b * EB + EPW * pw;           // first element of this half
-  const int ne = max(0, min(EPW, K - e0));    // live elements of this half
-  const int el = EPW * pw;                    // first slot column of this half
-
-  constexpr int NIU = (EPW * Np * 8 + 1023) / 1024;
-  constexpr int NIG = (EPW * GEO_STRIDE * 8 + 1023) / 1024;
-#pragma unroll
-  for (int v = 0; v < 5; v++) dma_copy<NIU>(a.u_in + v * KNp + (size_t)e0 * Np, &sl.u[v][el * Np], ne * Np * 8, lane);
-  dma_copy<NIG>(a.geo + (size_t)e0 * GEO_STRIDE, &sl.geo[el * GEO_STRIDE], ne * GEO_STRIDE * 8, lane);
-
-  double lam_own = 0.0, lam_nbr = 0.0;
-  if (lane < EPW * 4 && lane < ne * 4) {
-    const int cc = cb_cur[el * 4 + lane];
-    lam_own = a.maxv[(size_t)e0 * 4 + lane];
-    lam_nbr = a.maxv[(size_t)conn_nbr(cc) * 4 + conn_face(cc)];
+  const int e0 = b * EB;
+  const int ne = min(EB, K - e0);
+  if (pw == 0) {
+    pr.lam_own = 0.0;
+    pr.lam_nbr = 0.0;
+    if (lane < ne * 4) {
+      const int cc = cb_cur[lane];
+      pr.lam_own = a.maxv[(size_t)e0 * 4 + lane];
+      pr.lam_nbr = a.maxv[(size_t)conn_nbr(cc) * 4 + conn_face(cc)];
+    }
+    pr.cn = 0;
+    const int ne_n = nb >= 0 ? min(EB, K - nb * EB) : 0;
+    if (lane < ne_n * 4) pr.cn = a.conn[(size_t)nb * EB * 4 + lane];
   }
-  int cn = 0;
-  const int ne_n = nb >= 0 ? max(0, min(EPW, K - (nb * EB + EPW * pw))) : 0;
-  if (lane < ne_n * 4) cn = a.conn[(size_t)(nb * EB + EPW * pw) * 4 + lane];
-
-  // exterior traces: pair P = lane + 64 s is (element el + P / NFF4, face node P % NFF4)
   const double *ubase[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) ubase[v] = a.u_in + v * KNp;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
-    const int P = lane + 64 * s;
+    const int pl = lane + 64 * s, P = PPW * pw + pl;
     const int q = P / NFF4, m = P - q * NFF4;
-    const bool ok = P < EPW * NFF4 && q < ne && m < NFF;
+    // dead pairs load node 0 (never used): no select on the loaded values,
+    // so nothing here waits for the loads
     int pidx = 0;  // K*Np < 2^31 (checked on the host)
-    if (ok) {
+    if (pl < PPW && P < S::NPAIR && q < ne && m < NFF) {
       const int f = m / Nfp, jf = m - f * Nfp;
-      const int cc = cb_cur[(el + q) * 4 + f];
+      const int cc = cb_cur[q * 4 + f];
       pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) tr.up[s][v] = ok ? ubase[v][pidx] : (v == 0 ? 1.0 : 0.0);
+    for (int v = 0; v < 5; v++) pr.up[s][v] = ubase[v][pidx];
   }
+}
 
-  // ---- every load and DMA copy has landed ----
-  __builtin_amdgcn_s_waitcnt(0);
-  if (lane < EPW * 4) {
-    sl.tau[el * 4 + lane] = 0.25 * (lam_own + lam_nbr) * a.tau_scale;
-    cb_next[el * 4 + lane] = cn;
-  }
-  // columns past the last element: a harmless state (rho = 1, at rest)
-  for (int i = ne * Np + lane; i < EPW * Np; i += 64) {
+template <int N>
+__device__ inline void ws_dma(const StageArgs &a, WsSlot<N> &sl, int b, int pw, int lane) {
+  using S = WsShape<N>;
+  constexpr int Np = S::Np, EB = S::EB, EH = S::EH;
+  const size_t KNp = (size_t)a.K * Np;
+  constexpr int NIU = (EH * Np * 8 + 1023) / 1024;
+  constexpr int NIG = (EH * GEO_STRIDE * 8 + 1023) / 1024;
 #pragma unroll
-    for (int v = 0; v < 5; v++) sl.u[v][el * Np + i] = v == 0 ? 1.0 : 0.0;
+  for (int x = 0; x < 2; x++) {
+    const int c = 2 * pw + x;  // copy c: variable c / 2 (c < 10) or the geometry (c >= 10), half c % 2
+    const int h = c & 1, el = EH * h, e0 = b * EB + el;
+    const int ne = max(0, min(EH, a.K - e0));
+    if (c < 10) {
+      const int v = c >> 1;
+      dma_copy<NIU>(a.u_in + v * KNp + (size_t)e0 * Np, &sl.u[v][el * Np], ne * Np * 8, lane);
+      // columns past the last element: a harmless state (rho = 1, at rest)
+      for (int i = ne * Np + lane; i < EH * Np; i += 64) sl.u[v][el * Np + i] = v == 0 ? 1.0 : 0.0;
+    } else {
+      dma_copy<NIG>(a.geo + (size_t)e0 * GEO_STRIDE, &sl.geo[el * GEO_STRIDE], ne * GEO_STRIDE * 8, lane);
+      for (int i = ne * GEO_STRIDE + lane; i < EH * GEO_STRIDE; i += 64) sl.geo[el * GEO_STRIDE + i] = 0.0;
+    }
   }
-  for (int i = ne * GEO_STRIDE + lane; i < EPW * GEO_STRIDE; i += 64) sl.geo[el * GEO_STRIDE + i] = 0.0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N>
+__device__ inline void ws_land(const StageArgs &a, WsSlot<N> &sl, int *cb_next, int pw, int lane,
+                               const WsProd<N> &pr) {
+  // ---- every load and DMA copy of this wave has landed ----
+  __builtin_amdgcn_s_waitcnt(0);
+  if (pw == 0) {
+    sl.tau[lane] = 0.25 * (pr.lam_own + pr.lam_nbr) * a.tau_scale;
+    cb_next[lane] = pr.cn;
+  }
 }
 
 // Pressure and contravariant velocity V_d = sum_x g_dx v_x of every node of
-// a staged batch; threads t = 0..nthr-1 (the consumer waves, whose VALU is
-// free while their MFMA chains run).
+// a staged batch; threads t = 0..nthr-1.
 template <int N>
 __device__ inline void ws_primitives(const StageArgs &a, WsSlot<N> &sl, int t, int nthr) {
   using S = WsShape<N>;
@@ -240,27 +287,25 @@ __device__ inline void ws_primitives(const StageArgs &a, WsSlot<N> &sl, int t, i
   }
 }
 
-// Producer phase 2: LM (or Rusanov) surface flux of the five variables at each
-// face node of this wave's half of batch b, times Fscale, into fs
-// ([variable][face node][element], row stride FS).
+// LM (or Rusanov) surface flux of the five variables times Fscale at this
+// producer's face-node pairs of batch b, into fs ([variable][face node]
+// [element], row stride FS); padding entries become zero.
 template <int N>
-__device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl, double *fs, const int *s_fmx, int b,
-                                        int pw, int lane_in, const WsTraces<N> &tr) {
+__device__ inline void ws_surface(const StageArgs &a, const WsSlot<N> &sl, double *fs, const int *s_fmx, int b,
+                                  int pw, int lane_in, const WsProd<N> &pr) {
   using S = WsShape<N>;
-  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, EPW = S::EPW, NFF4 = S::NFF4;
-  constexpr int NRS = WsTraces<N>::NRS;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4, PPW = S::PPW;
+  constexpr int NRS = WsProd<N>::NRS;
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
-  const int e0 = b * EB + EPW * pw;
-  const int ne = max(0, min(EPW, a.K - e0));
-  const int el = EPW * pw;
+  const int ne = min(EB, a.K - b * EB);
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
-    const int P = lane + 64 * s;
-    if (P < EPW * NFF4) {
-      const int q = P / NFF4, m = P - q * NFF4, e = el + q;
+    const int pl = lane + 64 * s, P = PPW * pw + pl;
+    if (pl < PPW && P < S::NPAIR) {
+      const int e = P / NFF4, m = P - e * NFF4;
       double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      if (q < ne && m < NFF) {
+      if (e < ne && m < NFF) {
         const int f = m / Nfp, jf = m - f * Nfp;
         const int o = e * Np + s_fmx[(f * 4 + 3) * Nfp + jf];
         const double *g = &sl.geo[e * GEO_STRIDE + 9 + f * 4];
@@ -268,14 +313,14 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
 #pragma unroll
         for (int v = 0; v < 5; v++) uM[v] = sl.u[v][o];
         if (a.flux == 1) {
-          euler_rusanov5(uM, tr.up[s], g, a.gamma, fl);
+          euler_rusanov5(uM, pr.up[s], g, a.gamma, fl);
 #pragma unroll
           for (int v = 0; v < 5; v++) fl[v] *= g[3];
         } else {
           const double tau = sl.tau[e * 4 + f];
           double ub[5];
 #pragma unroll
-          for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + tr.up[s][v]);
+          for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + pr.up[s][v]);
           const double inv = 1.0 / ub[0];
           const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
           const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
@@ -292,9 +337,7 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
 #pragma unroll
       for (int v = 0; v < 5; v++) fs[(v * NFF4 + m) * S::FS + e] = fl[v];
     }
-    // four face nodes in flight at a time: enough independent chains to
-    // cover the FP64 latencies, few enough to bound live registers
-    if (s % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_sched_barrier(0);  // one face node at a time: bounds live registers
   }
 }
 
@@ -304,8 +347,8 @@ __device__ inline void ws_stage_surface(const StageArgs &a, const WsSlot<N> &sl,
 // code).
 template <int N, int MODE, bool WREM>
 __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const double *fs, const double *s_arem,
-                                  const double *__restrict__ afrag, double dt, int b, int hi, int gsz, int v, int t,
-                                  int lane) {
+                                  const double *s_afs, const double *__restrict__ afrag, double dt, int b, int hi,
+                                  int gsz, int v, int t, int lane) {
   using S = WsShape<N>;
   using P = PrecTraits<PREC_F64>;
   constexpr int Np = S::Np, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV, NFF4 = S::NFF4;
@@ -317,14 +360,17 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
   const size_t KNp = (size_t)K * Np;
   const int kk = lane >> 4, e = lane & 15;
 
-  // operator rows of this wave, resident for the whole kernel (the empty asm
-  // makes them opaque, so they are never re-loaded inside the loop)
-  double afr[KS];
+  // operator rows of this wave for the volume k-steps, resident in registers
+  // for the whole kernel (the empty asm makes them opaque, so they are never
+  // re-loaded inside the loop); those of the surface k-steps are read from
+  // LDS (s_afs) at each use
+  double afr[3 * G];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) {
+  for (int ks = 0; ks < 3 * G; ks++) {
     afr[ks] = afrag[((size_t)t * KS + ks) * 64 + lane];
     asm volatile("" : "+v"(afr[ks]));
   }
+  const double *afs = s_afs + (size_t)t * KSS * 64 + lane;
   const double *fsv = fs + v * NFF4 * S::FS;
 
   for (int it = 0; b < hi; b += gsz, it++) {
@@ -360,7 +406,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
 #pragma unroll
     for (int ks = 0; ks < KSS; ks++) {
       const double bf = fsv[(4 * ks + kk) * S::FS + e];
-      acc = P::mfma(afr[3 * G + ks], bf, acc);
+      acc = P::mfma(afs[ks * 64], bf, acc);
 #pragma unroll
       for (int r = 0; r < REM; r++) {
         racc[r] += arw[r * KDP + KV + 4 * ks] * bf;
@@ -369,9 +415,6 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
       if (ks % 4 == 3) __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();  // surface buffer free for the next batch's fluxes
-
-    // primitives of the next batch (its node values landed in phase 1)
-    if (b + gsz < hi) ws_primitives<N>(a, sl[(it + 1) & 1], (v * RTF + t) * 64 + lane, S::NC * 64);
 
     // ---- phase 2: volume rows, f_d = c V_d + p g_d,v-1 ----
     double gv[3] = {0.0, 0.0, 0.0};
@@ -479,12 +522,13 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   __shared__ WsSlot<N> sl[2];
   __shared__ double s_fs[5 * S::NFF4 * S::FS];  // surface fluxes of the batch being consumed
   __shared__ double s_arem[REM > 0 ? REM * KDP : 1];
+  __shared__ double s_afs[RTF * S::KSS * 64];  // A-fragments of the surface k-steps, [tile][k-step][lane]
   __shared__ int s_fmx[16 * Nfp];
   __shared__ int s_cb[2][S::EB * 4];  // packed connectivity of the batch being staged / the next one
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform: role branches are scalar
+  const int w = __builtin_amdgcn_readfirstlane(ws_role<N>(tid >> 6));  // role, uniform: role branches are scalar
   const size_t KNp = (size_t)a.K * Np;
 
   if (MODE == 0 && !a.scal->active) {
@@ -496,6 +540,10 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
 
   for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
   for (int i = tid; i < REM * KDP; i += NT) s_arem[i] = arem[i];
+  for (int i = tid; i < RTF * S::KSS * 64; i += NT) {
+    const int tt = i / (S::KSS * 64), r = i - tt * S::KSS * 64;
+    s_afs[i] = afrag[((size_t)tt * S::KS + 3 * S::G) * 64 + r];
+  }
 
   // Batch schedule: group x = blockIdx % ngrp walks batches [lo, hi) with
   // the gridDim/ngrp workgroups of the group striding together.
@@ -503,42 +551,55 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   const int lo = (int)((long)nbatch * grp / ngrp), hi = (int)((long)nbatch * (grp + 1) / ngrp);
   int b = lo + gsl;
 
-  if (w >= NC && b < hi) {
+  if (w == NC && b < hi) {
     // connectivity of the first batch (later batches' is prefetched)
-    const int pw = w - NC, e0 = b * S::EB + S::EPW * pw;
-    if (lane < S::EPW * 4 && e0 * 4 + lane < a.K * 4) s_cb[0][S::EPW * pw * 4 + lane] = a.conn[(size_t)e0 * 4 + lane];
+    const int e0 = b * S::EB;
+    if (e0 * 4 + lane < a.K * 4) s_cb[0][lane] = a.conn[(size_t)e0 * 4 + lane];
   }
-  __syncthreads();  // s_fmx, s_arem, s_cb
+  __syncthreads();  // s_fmx, s_arem, s_afs, s_cb
 
   if (w >= NC) {
-    // producer waves: batch b's staging precedes the loop; in the loop's
-    // iteration for batch b they stage b + gsz.  Staging call n reads the
-    // connectivity buffer n & 1 and fills the other.
+    // producer waves.  Staging call n (batch lo + gsl + n*gsz) reads the
+    // connectivity buffer n & 1 and fills the other.  In the period of batch
+    // b they copy batch b + gsz into the free slot (phase 1), then form its
+    // primitives and surface fluxes and gather batch b + 2*gsz (phase 2).
     const int pw = w - NC;
-    WsTraces<N> tr;
-    if (b < hi) ws_stage_nodes<N>(a, sl[0], s_fmx, s_cb[0], s_cb[1], b, b + gsz < hi ? b + gsz : -1, pw, lane, tr);
-    __syncthreads();
-    if (b < hi) ws_stage_surface<N>(a, sl[0], s_fs, s_fmx, b, pw, lane, tr);
-    __syncthreads();
+    WsProd<N> pr;
+    if (b < hi) {
+      ws_gather<N>(a, s_fmx, s_cb[0], b, b + gsz < hi ? b + gsz : -1, pw, lane, pr);
+      ws_dma<N>(a, sl[0], b, pw, lane);
+      ws_land<N>(a, sl[0], s_cb[1], pw, lane, pr);
+    }
+    __syncthreads();  // batch b staged
+    if (b < hi) {
+      ws_primitives<N>(a, sl[0], pw * 64 + lane, S::NPW * 64);
+      ws_surface<N>(a, sl[0], s_fs, s_fmx, b, pw, lane, pr);
+    }
+    if (b + gsz < hi) ws_gather<N>(a, s_fmx, s_cb[1], b + gsz, b + 2 * gsz < hi ? b + 2 * gsz : -1, pw, lane, pr);
+    __syncthreads();  // batch b's primitives and surface fluxes complete
     for (int it = 0; b < hi; b += gsz, it++) {
       const int nb = b + gsz, nnb = b + 2 * gsz;
-      if (nb < hi)
-        ws_stage_nodes<N>(a, sl[(it + 1) & 1], s_fmx, s_cb[(it + 1) & 1], s_cb[it & 1], nb, nnb < hi ? nnb : -1, pw,
-                          lane, tr);
-      __syncthreads();  // consumers are done with the surface buffer
-      if (nb < hi) ws_stage_surface<N>(a, sl[(it + 1) & 1], s_fs, s_fmx, nb, pw, lane, tr);
+      if (nb < hi) {
+        ws_dma<N>(a, sl[(it + 1) & 1], nb, pw, lane);
+        ws_land<N>(a, sl[(it + 1) & 1], s_cb[it & 1], pw, lane, pr);
+      }
+      __syncthreads();  // consumers are done with the surface buffer; batch nb staged
+      if (nb < hi) {
+        ws_primitives<N>(a, sl[(it + 1) & 1], pw * 64 + lane, S::NPW * 64);
+        ws_surface<N>(a, sl[(it + 1) & 1], s_fs, s_fmx, nb, pw, lane, pr);
+      }
+      if (nnb < hi) ws_gather<N>(a, s_fmx, s_cb[it & 1], nnb, nnb + gsz < hi ? nnb + gsz : -1, pw, lane, pr);
       __syncthreads();
     }
     return;
   }
-  __syncthreads();  // batch b's nodes staged
-  if (b < hi) ws_primitives<N>(a, sl[0], tid, NC * 64);
-  __syncthreads();  // batch b's surface fluxes and primitives staged
+  __syncthreads();  // batch b staged
+  __syncthreads();  // batch b's primitives and surface fluxes complete
   const int v = w / RTF, t = w - v * RTF;
   if (REM > 0 && t == 0)
-    ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
+    ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, s_afs, afrag, dt, b, hi, gsz, v, t, lane);
   else
-    ws_consume<N, MODE, false>(a, sl, s_fs, s_arem, afrag, dt, b, hi, gsz, v, t, lane);
+    ws_consume<N, MODE, false>(a, sl, s_fs, s_arem, s_afs, afrag, dt, b, hi, gsz, v, t, lane);
 }
 
 // --------------------------------------------------------------------------
