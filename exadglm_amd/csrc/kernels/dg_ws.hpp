@@ -164,6 +164,8 @@ __device__ inline int ws_role(int hw) {
   }
 }
 
+__device__ inline int ws_batch(const int *order, int b) { return order ? order[b] : b; }
+
 // Registers a producer wave carries from one staging step to the next.
 template <int N>
 struct WsProd {
@@ -347,8 +349,8 @@ __device__ inline void ws_surface(const StageArgs &a, const WsSlot<N> &sl, doubl
 // code).
 template <int N, int MODE, bool WREM>
 __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const double *fs, const double *s_arem,
-                                  const double *s_afs, const double *__restrict__ afrag, double dt, int b, int hi,
-                                  int gsz, int v, int t, int lane) {
+                                  const double *s_afs, const double *__restrict__ afrag, const int *order, double dt,
+                                  int b, int hi, int gsz, int v, int t, int lane) {
   using S = WsShape<N>;
   using P = PrecTraits<PREC_F64>;
   constexpr int Np = S::Np, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV, NFF4 = S::NFF4;
@@ -375,7 +377,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
 
   for (int it = 0; b < hi; b += gsz, it++) {
     const WsSlot<N> &s = sl[it & 1];
-    const int e0 = b * EB;
+    const int e0 = ws_batch(order, b) * EB;
     const int ne = min(EB, K - e0);
     const bool live = e < ne;
     const size_t gbase = v * KNp + (size_t)(e0 + (live ? e : 0)) * Np;
@@ -512,9 +514,12 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
 //   MODE 0: LSRK update; MODE 1: write the RHS only.
 // ngrp: number of XCD groups the grid is split into (8, or 1 for plain
 // striding); gridDim.x must be a multiple of ngrp.
+// order: batch visiting order (physical batch order[b] for schedule position b;
+// null: b itself).
 template <int N, int MODE>
 __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, const double *__restrict__ afrag,
-                                                                   const double *__restrict__ arem, int nbatch,
+                                                                   const double *__restrict__ arem,
+                                                                   const int *__restrict__ order, int nbatch,
                                                                    int ngrp) {
   using S = WsShape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, KDP = S::KDP, RTF = S::RTF, REM = S::REM, NC = S::NC, NT = S::NT;
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
 
   if (w == NC && b < hi) {
     // connectivity of the first batch (later batches' is prefetched)
-    const int e0 = b * S::EB;
+    const int e0 = ws_batch(order, b) * S::EB;
     if (e0 * 4 + lane < a.K * 4) s_cb[0][lane] = a.conn[(size_t)e0 * 4 + lane];
   }
   __syncthreads();  // s_fmx, s_arem, s_afs, s_cb
@@ -564,31 +569,33 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
     // b they copy batch b + gsz into the free slot (phase 1), then form its
     // primitives and surface fluxes and gather batch b + 2*gsz (phase 2).
     const int pw = w - NC;
+    // physical batch of schedule position x (-1: past the range)
+    auto pb = [&](int x) { return x < hi ? ws_batch(order, x) : -1; };
     WsProd<N> pr;
     if (b < hi) {
-      ws_gather<N>(a, s_fmx, s_cb[0], b, b + gsz < hi ? b + gsz : -1, pw, lane, pr);
-      ws_dma<N>(a, sl[0], b, pw, lane);
+      ws_gather<N>(a, s_fmx, s_cb[0], pb(b), pb(b + gsz), pw, lane, pr);
+      ws_dma<N>(a, sl[0], pb(b), pw, lane);
       ws_land<N>(a, sl[0], s_cb[1], pw, lane, pr);
     }
     __syncthreads();  // batch b staged
     if (b < hi) {
       ws_primitives<N>(a, sl[0], pw * 64 + lane, S::NPW * 64);
-      ws_surface<N>(a, sl[0], s_fs, s_fmx, b, pw, lane, pr);
+      ws_surface<N>(a, sl[0], s_fs, s_fmx, pb(b), pw, lane, pr);
     }
-    if (b + gsz < hi) ws_gather<N>(a, s_fmx, s_cb[1], b + gsz, b + 2 * gsz < hi ? b + 2 * gsz : -1, pw, lane, pr);
+    if (b + gsz < hi) ws_gather<N>(a, s_fmx, s_cb[1], pb(b + gsz), pb(b + 2 * gsz), pw, lane, pr);
     __syncthreads();  // batch b's primitives and surface fluxes complete
     for (int it = 0; b < hi; b += gsz, it++) {
       const int nb = b + gsz, nnb = b + 2 * gsz;
       if (nb < hi) {
-        ws_dma<N>(a, sl[(it + 1) & 1], nb, pw, lane);
+        ws_dma<N>(a, sl[(it + 1) & 1], pb(nb), pw, lane);
         ws_land<N>(a, sl[(it + 1) & 1], s_cb[it & 1], pw, lane, pr);
       }
       __syncthreads();  // consumers are done with the surface buffer; batch nb staged
       if (nb < hi) {
         ws_primitives<N>(a, sl[(it + 1) & 1], pw * 64 + lane, S::NPW * 64);
-        ws_surface<N>(a, sl[(it + 1) & 1], s_fs, s_fmx, nb, pw, lane, pr);
+        ws_surface<N>(a, sl[(it + 1) & 1], s_fs, s_fmx, pb(nb), pw, lane, pr);
       }
-      if (nnb < hi) ws_gather<N>(a, s_fmx, s_cb[it & 1], nnb, nnb + gsz < hi ? nnb + gsz : -1, pw, lane, pr);
+      if (nnb < hi) ws_gather<N>(a, s_fmx, s_cb[it & 1], pb(nnb), pb(nnb + gsz), pw, lane, pr);
       __syncthreads();
     }
     return;
@@ -597,9 +604,9 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   __syncthreads();  // batch b's primitives and surface fluxes complete
   const int v = w / RTF, t = w - v * RTF;
   if (REM > 0 && t == 0)
-    ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, s_afs, afrag, dt, b, hi, gsz, v, t, lane);
+    ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
   else
-    ws_consume<N, MODE, false>(a, sl, s_fs, s_arem, s_afs, afrag, dt, b, hi, gsz, v, t, lane);
+    ws_consume<N, MODE, false>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
 }
 
 // --------------------------------------------------------------------------

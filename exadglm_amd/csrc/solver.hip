@@ -9,6 +9,7 @@
 // euler_nopad_dglm.cu:79-96).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -72,6 +73,7 @@ struct dglm_ctx {
   int mfma_grid = 0;              // persistent grid size (0 = not yet sized)
   double *afrag_ws = nullptr;     // operator rows in the warp-specialised kernel's k order
   double *arem_ws = nullptr;      // rows left to the VALU by that kernel
+  int *ws_order = nullptr;        // its batch visiting order (null: mesh order)
   int ws_grid = 0, ws_groups = 1;  // its persistent grid and XCD groups
   struct dglm_adj_tape *tape = nullptr;  // adjoint buffers and recorded trajectory
   bool use_graph = true;                 // replay two-step hipGraphs in dglm_step
@@ -278,7 +280,7 @@ void launch_ws(dglm_ctx *c, const StageArgs &a) {
       groups = 1;
     }
     hipLaunchKernelGGL((stage_ws_kernel<N, MODE>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->afrag_ws,
-                       c->arem_ws, nbatch, groups);
+                       c->arem_ws, c->ws_order, nbatch, groups);
   }
 }
 
@@ -382,7 +384,8 @@ void free_ctx(dglm_ctx *c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
-                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx, c->afrag_ws, c->arem_ws};
+                  c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx, c->afrag_ws, c->arem_ws,
+                  c->ws_order};
   for (void *q : ptrs)
     if (q) hipFree(q);
   free_tape(c->tape);
@@ -565,6 +568,43 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
     }
   }
 
+  // Batch order of the warp-specialised kernel: 16-element batches sorted by
+  // the Morton code of their node centroid, so that the workgroups of one XCD,
+  // which walk a contiguous stretch of the order together, work on a compact
+  // region and find the neighbours' face traces in their L2.
+  std::vector<int> ws_order;
+  if (!afr_ws.empty() && m->px && m->py && m->pz) {
+    const int nb = (K + 15) / 16;
+    const size_t KNp = (size_t)K * Np;
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    const double *pc[3] = {m->px, m->py, m->pz};
+    std::vector<double> cen((size_t)nb * 3, 0.0);
+    for (int d = 0; d < 3; d++)
+      for (size_t i = 0; i < KNp; i++) {
+        const double x = pc[d][i];
+        lo[d] = x < lo[d] ? x : lo[d];
+        hi[d] = x > hi[d] ? x : hi[d];
+        cen[(i / Np / 16) * 3 + d] += x;
+      }
+    std::vector<std::pair<unsigned long long, int>> key(nb);
+    for (int bi = 0; bi < nb; bi++) {
+      const int ne = std::min(16, K - 16 * bi);
+      unsigned long long code = 0;
+      unsigned q[3];
+      for (int d = 0; d < 3; d++) {
+        const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
+        const double t = (cen[(size_t)bi * 3 + d] / ((double)ne * Np) - lo[d]) / span;
+        q[d] = (unsigned)std::min(1023.0, std::max(0.0, t * 1024.0));
+      }
+      for (int bit = 9; bit >= 0; bit--)
+        for (int d = 0; d < 3; d++) code = (code << 1) | ((q[d] >> bit) & 1u);
+      key[bi] = {code, bi};
+    }
+    std::stable_sort(key.begin(), key.end());
+    ws_order.resize(nb);
+    for (int bi = 0; bi < nb; bi++) ws_order[bi] = key[bi].second;
+  }
+
   dglm_ctx *c = new dglm_ctx();
   c->p = *p;
   c->K = K;
@@ -593,6 +633,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (afrag_bytes && (rc = dev_alloc((char **)&c->afrag, afrag_bytes, "A-fragments"))) ||
       (!afr_ws.empty() && (rc = dev_alloc(&c->afrag_ws, afr_ws.size(), "A-fragments (warp-specialised)"))) ||
       (!arem_ws.empty() && (rc = dev_alloc(&c->arem_ws, arem_ws.size(), "remainder rows"))) ||
+      (!ws_order.empty() && (rc = dev_alloc(&c->ws_order, ws_order.size(), "batch order"))) ||
       (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "affine-offset tables")))) {
     free_ctx(c);
     return rc;
@@ -618,6 +659,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
                   : hipSuccess,
       c->arem_ws ? hipMemcpy(c->arem_ws, arem_ws.data(), arem_ws.size() * sizeof(double), hipMemcpyHostToDevice)
                  : hipSuccess,
+      c->ws_order ? hipMemcpy(c->ws_order, ws_order.data(), ws_order.size() * sizeof(int), hipMemcpyHostToDevice)
+                  : hipSuccess,
       c->rowsum ? hipMemcpy(c->rowsum, rsum.data(), rsum.size() * sizeof(double), hipMemcpyHostToDevice)
                 : hipSuccess,
       hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
