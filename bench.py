@@ -104,12 +104,35 @@ def run_forward(pb, u0, precision="fp64", flux="rusanov", steps=200, warmup=10, 
     return {"wall": wall, "u": u, "t": t, "kernel": name, "timing": tm}
 
 
+def profiled_traffic(ksym, args):
+    """HBM bytes per launch of the stage kernel from the committed rocprofv3
+    summary of this workload (FETCH_SIZE x 2 + WRITE_SIZE x 1 KiB per dispatch,
+    factors from a plain copy of known size), when the summary covers this
+    kernel and configuration; (None, None) otherwise."""
+    path = os.path.join("profiles", f"stage_ws_kernel_euler_N{args.order}_K{6 * args.nhex ** 3}.txt")
+    if ksym != f"stage_ws_kernel<{args.order}, 0>" or args.precision != "fp64" or \
+            not os.path.exists(os.path.join(ROOT, path)):
+        return None, None
+    in_table = False  # the memory-counter table: ... disp, FETCH KiB, WRITE KiB, HBM MB
+    for line in open(os.path.join(ROOT, path)):
+        if "FETCH KiB" in line and "WRITE KiB" in line:
+            in_table = True
+        elif in_table and line.startswith(f"void dglm::stage_ws_kernel<{args.order}, 0>"):
+            f = line.split()
+            try:
+                fetch, write = float(f[-3]), float(f[-2])
+            except ValueError:
+                return None, None
+            if fetch > 0 and write > 0:
+                return (2 * fetch + write) * 1024, path + " (rocprofv3 --pmc FETCH_SIZE x2 + WRITE_SIZE x1 KiB)"
+    return None, None
+
+
 def per_var_err(u, ref):
     """Relative L2 per conserved variable; rho*w (zero in the exact z-extruded
     vortex) is measured against the norm of the whole momentum."""
-    e = [float(np.linalg.norm(u[v] - ref[v]) / np.linalg.norm(ref[v])) for v in range(5)]
-    e[3] = float(np.linalg.norm(u[3] - ref[3]) / np.linalg.norm(ref[1:4]))
-    return e
+    den = [np.linalg.norm(ref[1:4]) if v == 3 else np.linalg.norm(ref[v]) for v in range(5)]
+    return [float(np.linalg.norm(u[v] - ref[v]) / den[v]) for v in range(5)]
 
 
 def mixed_precision_leg(order=6, nhex=24, steps=490, warmup=10):
@@ -195,7 +218,16 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a profiled run to bytes)")
+    ap.add_argument("--adjoint-only", type=int, default=0, metavar="STEPS",
+                    help="only run the FP64 Euler adjoint leg with STEPS steps (for a profiler) and print its times")
     args = ap.parse_args()
+    if args.adjoint_only:
+        res = adjoint_leg("fp64", steps=args.adjoint_only)
+        res.pop("grad")
+        res["ratio_record_plus_backward_over_forward"] = (res["t_record_s"] + res["t_backward_s"]) / \
+            res["t_forward_s"]
+        print(json.dumps(res), flush=True)
+        return
     if args.calibrate_copy:
         import torch
 
@@ -238,8 +270,10 @@ def main():
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     else:
         roof = {"bound": "mfma", "achieved": tfl, "peak": pk, "unit": "TFLOP/s", "frac": tfl / pk}
-    roof["traffic"] = None
-    roof["kernel"] = f"stage_{kernel_name}_kernel<{args.order},EULER,0>"
+    ksym = f"stage_ws_kernel<{args.order}, 0>" if kernel_name == "mfma_ws" else \
+        f"stage_{kernel_name}_kernel<{args.order}, EULER, 0>"
+    roof["traffic"], roof["traffic_source"] = profiled_traffic(ksym, args)
+    roof["kernel"] = ksym
     roof["avg_launch_us"] = st_avg_s * 1e6
     roof["algorithmic_bytes_per_launch"] = B
     roof["contraction_flops_per_launch"] = F
