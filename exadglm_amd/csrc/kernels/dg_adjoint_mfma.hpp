@@ -99,10 +99,16 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
   for (int i = tid; i < Np * 3; i += NT) s_slots[i] = a.node_slots[i];
   for (int i = tid; i < ne * GEO_STRIDE; i += NT) s_geo[i] = a.geo[(size_t)e0 * GEO_STRIDE + i];
-  for (int i = tid; i < ne * 4; i += NT) {
-    const int c = a.conn[(size_t)e0 * 4 + i];
-    s_conn[i] = c;
-    s_tau[i] = 0.25 * (a.maxv[(size_t)e0 * 4 + i] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) * a.tau_scale;
+  // wave speeds for tau: loaded here, written to LDS after the contraction
+  // (tau is first read in phase 2), so their dependent loads do not hold up
+  // phase 0
+  static_assert(EB * 4 <= 64, "one face side per lane of wave 0");
+  double lam_own = 0.0, lam_nbr = 0.0;
+  if (tid < ne * 4) {
+    const int c = a.conn[(size_t)e0 * 4 + tid];
+    s_conn[tid] = c;
+    lam_own = a.maxv[(size_t)e0 * 4 + tid];
+    lam_nbr = a.maxv[conn_nbr(c) * 4 + conn_face(c)];
   }
   for (int v = 0; v < 5; v++) {
     for (int i = tid; i < NpA * EB; i += NT) s_r[v][i] = 0.0;  // padded rows and columns
@@ -157,6 +163,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
       }
     }
   }
+  if (tid < ne * 4) s_tau[tid] = 0.25 * (lam_own + lam_nbr) * a.tau_scale;
   __syncthreads();
 
   // ---- phase 2: transposed Jacobians at the thread's volume node ----
