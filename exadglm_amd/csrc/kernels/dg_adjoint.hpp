@@ -349,28 +349,47 @@ __global__ __launch_bounds__(256) void adjoint_dt_reduce(const double *__restric
 // of face f and, for the face attaining max(lambda*Fscale), dt.  The
 // derivative of lambda = sqrt(max v^2) + sqrt(max |gamma p/rho|) follows the
 // argmax nodes the forward picked (first strict maximum, as calc_maxv).
-// One thread per face side (64 elements x 4 faces per block) finds the argmax
-// nodes and forms their ubar increments into LDS; one thread per element then
-// adds them in face order, so the accumulation order per node is fixed and
-// the result deterministic.
-constexpr int LAM_ADJ_EB = 64;
+// A block takes LAM_ADJ_EB(Np) elements: |v|^2 and |gamma p/rho| of their
+// nodes are formed with coalesced loads into LDS (the arithmetic of the
+// forward wave-speed kernel, so the same argmax), then one thread per face side
+// scans its face nodes and forms the ubar increments at the argmax nodes into
+// LDS, and four threads per element (one or two variables each) add them in
+// face order, so the accumulation order per node is fixed and the result
+// deterministic.
+__host__ __device__ constexpr int lam_adj_elems(int Np) { return Np <= 35 ? 64 : 32; }
 
 template <int N>
-__global__ __launch_bounds__(4 * LAM_ADJ_EB) void lambda_adjoint_kernel(
+__global__ __launch_bounds__(4 * lam_adj_elems(Ord<N>::Np)) void lambda_adjoint_kernel(
     int K, const double *__restrict__ u, const double *__restrict__ lam, const double *__restrict__ taubar,
     const int *__restrict__ conn, const int *__restrict__ fmx, const double *dt_seq, const int *fstar_seq, int step,
     const double *scal, double tau_scale, double gamma, double *ubar) {
   using O = Ord<N>;
   constexpr int Np = O::Np, Nfp = O::Nfp;
-  constexpr int EB = LAM_ADJ_EB;
-  __shared__ int s_nv[4 * EB], s_nc[4 * EB];  // argmax volume nodes (-1: none)
-  __shared__ double s_dv[4][4 * EB];          // increments of rho, rho v at the |v|^2 argmax
-  __shared__ double s_dc[5][4 * EB];          // increments of all five at the c^2 argmax
+  constexpr int EB = lam_adj_elems(Np), NT = 4 * EB;
+  __shared__ double s_v2[EB * Np], s_c2[EB * Np];
+  __shared__ int s_fm[4 * Nfp];
+  __shared__ int s_nv[NT], s_nc[NT];  // argmax volume nodes (-1: none)
+  __shared__ double s_dv[4][NT];      // increments of rho, rho v at the |v|^2 argmax
+  __shared__ double s_dc[5][NT];      // increments of all five at the c^2 argmax
   const size_t KNp = (size_t)K * Np;
   const int t = threadIdx.x;
-  const int e = blockIdx.x * EB + (t >> 2), f = t & 3;
+  const int e0 = blockIdx.x * EB;
+  const int ne = min(EB, K - e0);
+  for (int i = t; i < 4 * Nfp; i += NT) s_fm[i] = fmx[((i / Nfp) * 4 + 3) * Nfp + i % Nfp];
+  for (int i = t; i < ne * Np; i += NT) {
+    const size_t idx = (size_t)e0 * Np + i;
+    const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx],
+                 u5 = u[4 * KNp + idx];
+    const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double pres = (gamma - 1.0) * (u5 - 0.5 * u1 * v2);
+    s_v2[i] = v2;
+    s_c2[i] = fabs(gamma * pres / u1);
+  }
+  __syncthreads();
+  const int el = t >> 2, f = t & 3, e = e0 + el;
   int nv = -1, nc = -1;
-  if (e < K) {
+  if (el < ne) {
     const double dtbar = scal[1], dt = dt_seq[step];
     const int o = e * 4 + f;
     const int c = conn[o];
@@ -378,17 +397,11 @@ __global__ __launch_bounds__(4 * LAM_ADJ_EB) void lambda_adjoint_kernel(
     double lb = 0.25 * tau_scale * (taubar[o] + taubar[nbf]);
     if (o == fstar_seq[step]) lb += -dtbar * dt / lam[o];
     if (lb != 0.0) {
-      const int *fm = fmx + (f * 4 + 3) * Nfp;
       double mv2 = 0.0, mc2 = 0.0;
       int jv = -1, jc = -1;
-#pragma unroll 3
       for (int j = 0; j < Nfp; j++) {
-        const size_t idx = (size_t)e * Np + fm[j];
-        const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx],
-                     u5 = u[4 * KNp + idx];
-        const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
-        const double v2 = vx * vx + vy * vy + vz * vz;
-        const double c2 = fabs(gamma * ((gamma - 1.0) * (u5 - 0.5 * u1 * v2)) / u1);
+        const int q = el * Np + s_fm[f * Nfp + j];
+        const double v2 = s_v2[q], c2 = s_c2[q];
         if (v2 > mv2) {
           mv2 = v2;
           jv = j;
@@ -399,7 +412,7 @@ __global__ __launch_bounds__(4 * LAM_ADJ_EB) void lambda_adjoint_kernel(
         }
       }
       if (jv >= 0) {
-        nv = fm[jv];
+        nv = s_fm[f * Nfp + jv];
         const size_t idx = (size_t)e * Np + nv;
         const double r = u[idx], v[3] = {u[KNp + idx] / r, u[2 * KNp + idx] / r, u[3 * KNp + idx] / r};
         const double s = lb / (2.0 * sqrt(mv2));
@@ -407,7 +420,7 @@ __global__ __launch_bounds__(4 * LAM_ADJ_EB) void lambda_adjoint_kernel(
         for (int q = 0; q < 3; q++) s_dv[1 + q][t] = s * (2.0 * v[q] / r);
       }
       if (jc >= 0) {
-        nc = fm[jc];
+        nc = s_fm[f * Nfp + jc];
         const size_t idx = (size_t)e * Np + nc;
         const double r = u[idx], v[3] = {u[KNp + idx] / r, u[2 * KNp + idx] / r, u[3 * KNp + idx] / r};
         const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -424,20 +437,20 @@ __global__ __launch_bounds__(4 * LAM_ADJ_EB) void lambda_adjoint_kernel(
   s_nv[t] = nv;
   s_nc[t] = nc;
   __syncthreads();
-  if (t < EB) {
-    const int el = blockIdx.x * EB + t;
-    if (el < K)
+  // thread (element el, k) adds variable k (and variable 4 for k = 0) over
+  // the element's four faces in face order; distinct variables never alias
+  if (el < ne) {
+    const int k = f;
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const int v = x == 0 ? k : 4;
+      if (x == 1 && k != 0) break;
       for (int ff = 0; ff < 4; ff++) {
-        const int q = 4 * t + ff;
-        if (s_nv[q] >= 0) {
-          const size_t idx = (size_t)el * Np + s_nv[q];
-          for (int v = 0; v < 4; v++) ubar[v * KNp + idx] += s_dv[v][q];
-        }
-        if (s_nc[q] >= 0) {
-          const size_t idx = (size_t)el * Np + s_nc[q];
-          for (int v = 0; v < 5; v++) ubar[v * KNp + idx] += s_dc[v][q];
-        }
+        const int q = 4 * el + ff;
+        if (v < 4 && s_nv[q] >= 0) ubar[v * KNp + (size_t)e * Np + s_nv[q]] += s_dv[v][q];
+        if (s_nc[q] >= 0) ubar[v * KNp + (size_t)e * Np + s_nc[q]] += s_dc[v][q];
       }
+    }
   }
 }
 

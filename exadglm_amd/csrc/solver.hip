@@ -1186,8 +1186,8 @@ void launch_adj_a_mfma(dglm_ctx *c, const AdjArgs &a) {
 template <int N>
 void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step) {
   dglm_adj_tape *T = c->tape;
-  hipLaunchKernelGGL((lambda_adjoint_kernel<N>), dim3((c->K + LAM_ADJ_EB - 1) / LAM_ADJ_EB),
-                     dim3(4 * LAM_ADJ_EB), 0, c->stream, c->K, u, lam,
+  constexpr int EB = lam_adj_elems(Ord<N>::Np);
+  hipLaunchKernelGGL((lambda_adjoint_kernel<N>), dim3((c->K + EB - 1) / EB), dim3(4 * EB), 0, c->stream, c->K, u, lam,
                      T->taubar, c->conn, c->fmx, T->dt, T->fstar, step, T->tsc, c->p.tau_scale, c->p.gamma,
                      T->ubar);
 }
