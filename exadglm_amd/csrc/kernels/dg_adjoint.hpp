@@ -262,6 +262,13 @@ __global__ __launch_bounds__(256) void adjoint_pass_b(AdjArgs a) {
   using O = Ord<N>;
   constexpr int Np = O::Np, Nfp = O::Nfp, NFF = O::NFF;
   constexpr int NV = NVar<PHYS>::value;
+  // the per-order tables in LDS: only the connectivity and the partner
+  // values remain global loads, two round trips per node
+  __shared__ int s_slots[Np * 3], s_fmx[16 * Nfp], s_fpos[4 * Np];
+  for (int i = threadIdx.x; i < Np * 3; i += 256) s_slots[i] = a.node_slots[i];
+  for (int i = threadIdx.x; i < 16 * Nfp; i += 256) s_fmx[i] = a.fmx[i];
+  for (int i = threadIdx.x; i < 4 * Np; i += 256) s_fpos[i] = a.fpos[i];
+  __syncthreads();
   const int K = a.K;
   const size_t KNp = (size_t)K * Np;
   const int idx = blockIdx.x * 256 + threadIdx.x;
@@ -276,13 +283,13 @@ __global__ __launch_bounds__(256) void adjoint_pass_b(AdjArgs a) {
   double tau_n[3];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    ms[q] = a.node_slots[i * 3 + q];
+    ms[q] = s_slots[i * 3 + q];
     if (ms[q] >= 0) {
       const int f = ms[q] / Nfp, j = ms[q] - f * Nfp;
       const int c = a.conn[(size_t)e * 4 + f];
       const int nb = conn_nbr(c), nf = conn_face(c);
-      const int pnode = a.fmx[(nf * 4 + conn_code(c)) * Nfp + j];  // neighbour's volume node
-      const int jp = a.fpos[nf * Np + pnode];                       // its slot on face nf
+      const int pnode = s_fmx[(nf * 4 + conn_code(c)) * Nfp + j];  // neighbour's volume node
+      const int jp = s_fpos[nf * Np + pnode];                       // its slot on face nf
       pidx[q] = (size_t)nb * Np + pnode;
       gn[q] = &a.geo[(size_t)nb * GEO_STRIDE + 9 + nf * 4];
       tau_n[q] = 0.25 * (a.maxv[(size_t)nb * 4 + nf] + a.maxv[(size_t)e * 4 + f]) * a.tau_scale;
