@@ -1,36 +1,35 @@
 // Warp-specialised FP64 stage kernel for the Euler equations (gfx950), and
 // the LDS-staged wave-speed kernel.
 //
-// Stage kernel: one persistent workgroup per CU, batches of 16 elements, two
-// wave roles and two phases per batch (one barrier each):
+// Stage kernel: one persistent 1024-thread workgroup per CU (16 waves, four
+// per SIMD, so every wave fits in 128 VGPRs), batches of 16 elements in two
+// LDS slots, two wave roles and two phases per batch (one barrier each):
 //
-//   phase 1  consumers contract the surface rows of batch b (the LM surface
-//            fluxes, staged in a single LDS buffer); producers issue every
-//            global access of batch b+1 at once -- node values and geometry
-//            records go straight into the batch's LDS slot
-//            (global_load_lds_dwordx4, no registers), the exterior traces of
-//            all face nodes and the wave speeds for tau are gathered into
-//            registers with the connectivity prefetched a batch earlier --
-//            then write tau for b+1;
-//   phase 2  consumers form p and the contravariant velocity
-//            V_d = sum_x g_dx v_x of batch b+1 (VALU work beside their MFMA
-//            chains), contract the volume rows of batch b and apply the
-//            LSRK update; producers turn b+1's traces into the LM (or
-//            Rusanov) surface flux of all five variables at every face node,
-//            written once per face node into the surface buffer that the
-//            consumers finished reading in phase 1.
+//   phase 1  consumers contract the surface rows of batch b (the surface
+//            fluxes, staged in one LDS buffer); producers copy batch b+1's
+//            node values and geometry records straight into its LDS slot
+//            (global_load_lds_dwordx4, no registers), wait for those and for
+//            b+1's exterior traces (gathered a phase earlier), write tau;
+//   phase 2  consumers contract the volume rows of batch b and apply the
+//            LSRK update; producers form b+1's pressure and contravariant
+//            velocity V_d = sum_x g_dx v_x, its LM (or Rusanov) surface flux
+//            of all five variables at every face node (written once per face
+//            node into the surface buffer the consumers finished reading in
+//            phase 1), and issue the gathers of b+2's exterior traces.
 //
-// Consumer wave (v, t) owns row tile t (16 nodes) of variable v and holds its
-// operator rows as v_mfma_f64_16x16x4f64 A-fragments in registers.  Its
-// volume B-fragments are formed in registers: for variable v the metric
-// flux is  f_d = c V_d + p g_d,v-1  (c = rho, rho*u_v or E + p; the pressure
-// term only for the momenta), one node's value feeding its three direction
-// k-steps (contraction order k = d*Np4 + j).  The Np - 16*RTF rows that do not
-// fill a tile are accumulated on the VALU by the t = 0 waves (operator rows in
-// LDS, partial sums reduced across the four k-lanes with two shuffles)
-// instead of padding a whole MFMA tile.  The CUs of one XCD walk a contiguous
-// range of batches, so the traces shared by neighbouring batches are served
-// from that XCD's L2.
+// Consumer wave (v, t) owns row tile t (16 nodes) of variable v and keeps its
+// operator rows for the volume k-steps as v_mfma_f64_16x16x4f64 A-fragments
+// in registers; those of the surface k-steps are read from LDS.  Its volume
+// B-fragments are formed in registers: for variable v the metric flux is
+// f_d = c V_d + p g_d,v-1  (c = rho, rho*u_v or E + p; the pressure term only
+// for the momenta), one node's value feeding its three direction k-steps
+// (contraction order k = d*Np4 + j).  The Np - 16*RTF rows that do not fill a
+// tile are accumulated on the VALU by the t = 0 waves (operator rows in LDS,
+// partial sums reduced across the four k-lanes with two shuffles) instead of
+// padding a whole MFMA tile.  Batches are visited in an order table (Morton
+// order of their centroids, built on the host) and the CUs of one XCD walk a
+// contiguous stretch of it, so the traces shared by neighbouring batches are
+// served from that XCD's L2.
 //
 // Reference semantics as in dg_fp64.hpp (euler_dglm.cpp:6-302,
 // euler_time.cpp:15-38).
