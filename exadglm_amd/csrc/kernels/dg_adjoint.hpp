@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void adjoint_dt_reduce(const double *__restric
 // LDS, and four threads per element (one or two variables each) add them in
 // face order, so the accumulation order per node is fixed and the result
 // deterministic.
-__host__ __device__ constexpr int lam_adj_elems(int Np) { return Np <= 35 ? 64 : 32; }
+__host__ __device__ constexpr int lam_adj_elems(int Np) { return Np <= 35 ? 64 : (Np <= 84 ? 32 : 16); }
 
 template <int N>
 __global__ __launch_bounds__(4 * lam_adj_elems(Ord<N>::Np)) void lambda_adjoint_kernel(

@@ -614,7 +614,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
 // maxima over the face nodes.  Same arithmetic and tie rule as
 // wavespeed_kernel.
 // --------------------------------------------------------------------------
-__host__ __device__ constexpr int wavespeed_elems(int Np) { return Np <= 35 ? 64 : 32; }
+__host__ __device__ constexpr int wavespeed_elems(int Np) { return Np <= 35 ? 64 : (Np <= 84 ? 32 : 16); }
 
 template <int N, int PHYS>
 __global__ __launch_bounds__(256) void wavespeed_lds_kernel(int K, const double *__restrict__ u,

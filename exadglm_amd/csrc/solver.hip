@@ -307,12 +307,14 @@ void launch_stage(dglm_ctx *c, const StageArgs &a) {
     launch_ws<N, MODE>(c, a);
     return;
   }
-  if (kind == DGLM_STAGE_MFMA) {
-    if (c->p.precision == DGLM_FP32)
-      launch_mfma<N, PHYS, MODE, PREC_F32>(c, a);
-    else
-      launch_mfma<N, PHYS, MODE, PREC_F64>(c, a);
-    return;
+  if constexpr (N <= 6) {
+    if (kind == DGLM_STAGE_MFMA) {
+      if (c->p.precision == DGLM_FP32)
+        launch_mfma<N, PHYS, MODE, PREC_F32>(c, a);
+      else
+        launch_mfma<N, PHYS, MODE, PREC_F64>(c, a);
+      return;
+    }
   }
   constexpr int EB = eb_simple<N, PHYS>();
   int nblk = (c->K + EB - 1) / EB;
@@ -337,6 +339,10 @@ bool pick(int N, ws_fn &ws, st_fn &st0, st_fn &st1) {
     DGLM_CASE(4)
     DGLM_CASE(5)
     DGLM_CASE(6)
+    DGLM_CASE(7)
+    DGLM_CASE(8)
+    DGLM_CASE(9)
+    DGLM_CASE(10)
 #undef DGLM_CASE
     default:
       return false;
@@ -450,7 +456,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: mixed precision is built for N in [1, 6]");
   if (p->flux != DGLM_FLUX_LM && p->flux != DGLM_FLUX_RUSANOV)
     return fail(DGLM_ERR_ARG, "dglm_create: flux must be DGLM_FLUX_LM or DGLM_FLUX_RUSANOV");
-  if (p->N < 1 || p->N > 6) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: N must be in [1, 6]");
+  if (p->N < 1 || p->N > 10) return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: N must be in [1, 10]");
   if (m->K <= 0) return fail(DGLM_ERR_ARG, "dglm_create: K must be positive");
   if (!(p->cfl > 0) || !(p->tau_scale >= 0) || !(p->gamma > 1.0) || std::isnan(p->tmax))
     return fail(DGLM_ERR_ARG, "dglm_create: invalid cfl / gamma / tau_scale / tmax");
@@ -1141,6 +1147,10 @@ bool pick_adj_impl(int N, adj_fn &A, adj_fn &B, lam_fn &Lf, int &eb) {
     DGLM_ACASE(4)
     DGLM_ACASE(5)
     DGLM_ACASE(6)
+    DGLM_ACASE(7)
+    DGLM_ACASE(8)
+    DGLM_ACASE(9)
+    DGLM_ACASE(10)
 #undef DGLM_ACASE
     default:
       return false;

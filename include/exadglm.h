@@ -105,10 +105,11 @@ int dglm_mesh_spatial_h(int K, const int *EtoV, const double *VXYZ, double *h);
  * the generated common.h (dglm_3d_launcher.py:152-205) become dglm_params.
  * ------------------------------------------------------------------------- */
 typedef struct dglm_params {
-  int N;           /* polynomial order (NP, NFP follow)                  */
+  int N;           /* polynomial order (NP, NFP follow): 1..10 in FP64,  */
+                   /* 1..6 in mixed precision                           */
   int physics;     /* DGLM_ADVECTION (NVAR=1) or DGLM_EULER (NVAR=5)     */
   int precision;   /* DGLM_FP64 / DGLM_FP32 / DGLM_BF16X3               */
-  int flux;        /* DGLM_FLUX_LM                                       */
+  int flux;        /* DGLM_FLUX_LM or DGLM_FLUX_RUSANOV                  */
   double cfl;      /* CFL                                                */
   double gamma;    /* GAMMA                                              */
   double tau_scale;/* TAU_SCALE                                          */
@@ -180,10 +181,11 @@ int dglm_set_timing(dglm_ctx *ctx, int on);
 /* Stage-kernel selection (the reference picks its GPU variant by file prefix,
  * *_nopad_ / *_pad_, dglm_3d_launcher.py:176-205).  SIMPLE: LDS-staged FP64;
  * MFMA: one workgroup per 16-element batch, operator row tiles in registers
- * (all orders, FP64 and mixed precision); MFMA_WS: persistent warp-specialised
+ * (N <= 6, FP64 and mixed precision); MFMA_WS: persistent warp-specialised
  * FP64 Euler kernel (producer waves stage the next batch while consumer waves
  * run the MFMA contraction; N = 3, 4).  AUTO takes MFMA_WS where it exists,
- * else MFMA.  dglm_get_stage_kernel reports the kernel that runs. */
+ * else MFMA, else SIMPLE (N = 7..10).  dglm_get_stage_kernel reports the
+ * kernel that runs. */
 enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2, DGLM_STAGE_MFMA_WS = 3 };
 int dglm_set_stage_kernel(dglm_ctx *ctx, int variant);
 int dglm_get_stage_kernel(dglm_ctx *ctx, int *variant);

@@ -21,7 +21,7 @@ def checksum(a):
 
 def main():
     out = {}
-    for N in range(1, 7):
+    for N in range(1, 11):
         Np = (N + 1) * (N + 2) * (N + 3) // 6
         Nfp = (N + 1) * (N + 2) // 2
         hN = N + 2

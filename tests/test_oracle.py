@@ -21,7 +21,7 @@ def _need_ref(physics, N):
 
 
 @pytest.mark.parametrize("physics", PHYS)
-@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("N", list(range(1, 11)))
 def test_rhs_matches_compiled_reference(physics, N):
     _need_ref(physics, N)
     pb = cube_problem(N, 2)

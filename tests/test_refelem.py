@@ -105,7 +105,7 @@ def _checksum(a):
     return [float(np.sum(a * a)), float(np.sum(a * w))]
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("N", list(range(1, 11)))
 def test_against_reference_table_checksums(N):
     """Checksums of the reference's shipped dim3/data/bin_mats tables (fixture
     written by tests/golden/make_operator_fixture.py)."""
@@ -121,12 +121,14 @@ def test_against_reference_table_checksums(N):
 
 
 @pytest.mark.skipif(not has_reference(), reason="reference checkout not present")
-@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("N", list(range(1, 11)))
 def test_against_reference_tables(N):
     tet, o, f = ops(N)
     Np, Nfp = tet.Np, tet.Nfp
     d = os.path.join(REFERENCE, "dim3/data/bin_mats", f"N{N}")
+    # the Vandermonde conditioning grows with N: 3.2e-15 and 3.7e-15 at N = 9, 10
+    tol = 3e-15 if N <= 8 else 5e-15
     for name, mine in dict(rst=tet.rst, M=o.M, Dr=o.Dr, WDr=o.WDr, WDs=o.WDs, WDt=o.WDt, Emat=f.Emat,
                            LIFT=f.LIFT).items():
         ref = np.fromfile(os.path.join(d, f"{name}.bin")).reshape(mine.shape)
-        assert np.abs(ref - mine).max() <= 3e-15 * np.abs(ref).max(), name
+        assert np.abs(ref - mine).max() <= tol * np.abs(ref).max(), name
