@@ -200,12 +200,18 @@ class DGLM3DBase:
 
 class DGLM3DSingleCPU(DGLM3DBase):
     def init_mesh(self, mesh_info, verbose=1):
-        if mesh_info["type"] == "uniform":
+        if mesh_info["type"] == "gmsh":
+            from .gmsh import Gmsh3DSingle
+
+            elem = Gmsh3DSingle(mesh_info["file"])  # *.msh file generated by Gmsh (ASCII 2.2 or 4.1)
+            if verbose >= 1:
+                print(f"Init Gmsh file: {mesh_info['file']}")
+        elif mesh_info["type"] == "uniform":
             elem = Uniform3DSingle(*mesh_info["param"])
             if verbose >= 1:
                 print(f"Init uniform mesh: {mesh_info['param']}")
         else:
-            raise NotImplementedError("only 'uniform' meshes are built (the Gmsh readers are absent upstream too)")
+            raise ValueError(f"mesh type must be 'gmsh' or 'uniform', got {mesh_info['type']!r}")
         self.mesh = Mesh3D(self.op_face, elem)
         self.Nelem = self.mesh.Nelem
 

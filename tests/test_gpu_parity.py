@@ -271,15 +271,27 @@ def test_steps_both_stage_kernels(gpu, physics, N):
             assert e < 1e-12, variant
 
 
-def test_launcher_drop_in(gpu, tmp_path):
+@pytest.mark.parametrize("mesh", ["uniform", "gmsh"])
+def test_launcher_drop_in(gpu, tmp_path, mesh):
     """The notebook flow of ex1.advection.21.gpu.single.ipynb through the
-    launcher-compatible API: 809 steps and the published L2."""
-    _say("launcher drop-in")
+    launcher-compatible API: 809 steps and the published L2; with the same
+    periodic box also read from a Gmsh .msh file (MSH 4.1 with its periodic
+    node pairs)."""
+    _say(f"launcher drop-in ({mesh} mesh)")
     from exadglm_amd.launcher import DGLM3DSingleGPU
 
     dg = DGLM3DSingleGPU(None, str(tmp_path / "ex1"), 5, 1, 2, {"CFL": 0.8, "tau_scale": 1.2}, devid=0)
-    dg.init_mesh({"type": "uniform", "param": [7, 7, 7, 0, 1, 0, 1, 0, 1, {"x": "pbc", "y": "pbc", "z": "pbc"}]},
-                 verbose=0)
+    param = [7, 7, 7, 0, 1, 0, 1, 0, 1, {"x": "pbc", "y": "pbc", "z": "pbc"}]
+    if mesh == "uniform":
+        dg.init_mesh({"type": "uniform", "param": param}, verbose=0)
+    else:
+        from exadglm_amd.mesh import Uniform3DSingle
+        from test_gmsh import _periodic_pairs, write_msh
+
+        u = Uniform3DSingle(*param)
+        path = str(tmp_path / "box.msh")
+        write_msh(path, u.VXYZ, u.EtoV, (), _periodic_pairs(u.VXYZ, (0, 1, 0, 1, 0, 1), [0, 1, 2]), "4.1")
+        dg.init_mesh({"type": "gmsh", "file": path}, verbose=0)
     dg.make_work_dir()
     dg.save_to_binaries(matrix_order="f", verbose=0)
     dg.save_to_header(["advec_nopad_data_host.cu"], verbose=0)
