@@ -18,7 +18,8 @@
  *   lserk_stage  advec_time.cpp:15-26,              euler_time.cpp:15-38
  *   runRK        advec_time.cpp:29-97,              euler_time.cpp:41-107
  * with the same floating-point expression order, so that it agrees with the
- * compiled reference to rounding (pinned in tests/test_oracle_ref.py).
+ * compiled reference to rounding (pinned in tests/test_oracle.py).  Faces
+ * with EtoB == 1 take a prescribed ub (the reference's Dirichlet stub).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -89,6 +90,12 @@ static void update_ub(const oracle_problem *P, const double *u, double *ub) {
     int ei = (int)(idx / NFF), fi = (int)((idx / Nfp) % NFACE);
     size_t idxM = (size_t)ei * Np + P->Fmask[fi * Nfp + idx % Nfp];
     size_t idxP = (size_t)P->vmapP[idx];
+    if (P->gD && P->EtoB && P->EtoB[idx / Nfp] == 1) {
+      /* Dirichlet: the prescribed face state (euler_dglm.cpp:133-139,
+         advec_dglm.cpp:113-119 leave it as "ub = function of x,y,z,t") */
+      for (int v = 0; v < P->nvar; v++) ub[v * KNF + idx] = P->gD[v * KNF + idx];
+      continue;
+    }
     for (int v = 0; v < P->nvar; v++) ub[v * KNF + idx] = 0.5 * (u[v * KNp + idxM] + u[v * KNp + idxP]);
   }
 }
@@ -374,16 +381,18 @@ int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, co
     for (int j = 0; j < Nfp; j++) {
       size_t idx = (size_t)fidx * Nfp + j;
       size_t iM = (size_t)ei * Np + P->Fmask[fi * Nfp + j], iP = (size_t)P->vmapP[idx];
+      /* a Dirichlet face's ub is data: no derivative */
+      const int dir = P->gD && P->EtoB && P->EtoB[fidx] == 1;
       if (nv == 5) {
         double ub[5], dub[5], out[5];
         for (int v = 0; v < 5; v++) {
-          ub[v] = 0.5 * (u[v * KNp + iM] + u[v * KNp + iP]);
-          dub[v] = 0.5 * (du[v * KNp + iM] + du[v * KNp + iP]);
+          ub[v] = dir ? P->gD[v * KNF + idx] : 0.5 * (u[v * KNp + iM] + u[v * KNp + iP]);
+          dub[v] = dir ? 0.0 : 0.5 * (du[v * KNp + iM] + du[v * KNp + iP]);
         }
         euler_jv(ub, nx, ny, nz, dub, P->gamma, out);
         for (int v = 0; v < 5; v++) w.fLM[v * KNF + idx] = (out[v] - tau * (dub[v] - du[v * KNp + iM])) * fs;
       } else {
-        double dub = 0.5 * (du[iM] + du[iP]);
+        double dub = dir ? 0.0 : 0.5 * (du[iM] + du[iP]);
         double vn = nx * P->vx[iM] + ny * P->vy[iM] + nz * P->vz[iM];
         w.fLM[idx] = (vn * dub - tau * (dub - du[iM])) * fs;
       }

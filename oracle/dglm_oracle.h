@@ -20,6 +20,10 @@ typedef struct oracle_problem {
   const double *rx, *ry, *rz, *sx, *sy, *sz, *tx, *ty, *tz;
   const double *nx, *ny, *nz;
   const double *vx, *vy, *vz;
+  /* Dirichlet faces (EtoB == 1, the reference's stubbed branch of
+   * calc_fluxLM): ub = gD there, [nvar][K*4*Nfp]; NULL for none. */
+  const int *EtoB;
+  const double *gD;
 } oracle_problem;
 
 /* One RHS at state u ([nvar][K*Np]); maxv, tau [K*4] optional. */

@@ -24,7 +24,8 @@ class OracleProblem(C.Structure):
                  ("cfl", C.c_double), ("gamma", C.c_double), ("tau_scale", C.c_double), ("tmax", C.c_double)]
                 + [(n, PI) for n in ("vmapP", "vmapF", "Fmask")]
                 + [(n, PD) for n in ("WDr", "WDs", "WDt", "LIFT", "Fscale", "rx", "ry", "rz", "sx", "sy", "sz",
-                                     "tx", "ty", "tz", "nx", "ny", "nz", "vx", "vy", "vz")])
+                                     "tx", "ty", "tz", "nx", "ny", "nz", "vx", "vy", "vz")]
+                + [("EtoB", PI), ("gD", PD)])
 
 
 _lib = None
@@ -70,7 +71,17 @@ class Oracle:
             v = [np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (n,))) for c in velocity]
             self._keep["vel"] = v
             P.vx, P.vy, P.vz = _d(v[0]), _d(v[1]), _d(v[2])
+        if "EtoB" in a:
+            self._keep["EtoB"] = np.ascontiguousarray(a["EtoB"], dtype=np.int32)
+            P.EtoB = _i(self._keep["EtoB"])
         self.P = P
+
+    def set_dirichlet(self, g):
+        """Face states ub on the faces with EtoB == 1, [nvar][K*4*Nfp]
+        (the other entries are not read)."""
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(self.nvar, -1)
+        self._keep["gD"] = g
+        self.P.gD = _d(g)
 
     def rhs(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.nvar, -1)

@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import rel_err, smooth_state
+from helpers import noisy_state, rel_err, smooth_state
 from exadglm_amd.problem import cube_problem, ic_advection_sine, ic_euler_density_wave, l2_error
 import pyoracle as O
 
@@ -145,20 +145,25 @@ def test_advection_period_return():
     assert errs[1] < errs[0] / 8  # at least O(h^3)
 
 
+@pytest.mark.parametrize("dirichlet", [False, True])
 @pytest.mark.parametrize("physics", PHYS)
-def test_oracle_tangent_linear(physics):
+def test_oracle_tangent_linear(physics, dirichlet):
     """The oracle's tangent-linear RHS (the reference for the GPU adjoint's
     dot-product test) agrees with central differences of the oracle RHS at
-    frozen tau; for advection (linear in u) it equals the RHS of the
-    direction itself."""
-    pb = cube_problem(3, 2)
+    frozen tau; for advection (linear in u) and periodic faces it equals the
+    RHS of the direction itself.  With Dirichlet faces (x cut open, a fixed
+    prescribed ub there) the face state does not move with u."""
+    bc = {"x": "bc1", "y": "pbc", "z": "pbc"} if dirichlet else None
+    pb = cube_problem(3, 2, bc=bc)
     u = smooth_state(pb, physics, seed=41)
     rng = np.random.default_rng(3)
     du = rng.standard_normal(u.shape)
     o = O.Oracle(pb, physics)
+    if dirichlet:
+        o.set_dirichlet(u[:, pb.mesh.vmapM.ravel()] * (1 + 0.1 * rng.standard_normal((u.shape[0], 1))))
     tl = O.rhs_tl(o, u, du)
     _, _, tau = o.rhs(u)
-    if physics == "advection":
+    if physics == "advection" and not dirichlet:
         assert rel_err(tl, O.rhs_tau(o, du, tau)) < 1e-14
     h = 1e-5 * np.abs(u).max() / np.abs(du).max()
     fd = (O.rhs_tau(o, u + h * du, tau) - O.rhs_tau(o, u - h * du, tau)) / (2 * h)
@@ -212,3 +217,45 @@ def test_trajectory_tangent_linear_vs_fd():
         fdf = (J(u0 + h * d, coeffs) - J(u0 - h * d, coeffs)) / (2 * h)
         assert abs(tlf - fdf) <= 1e-6 * abs(fdf), (tlf, fdf)
         assert abs(tlf - fd) > 1e-4 * abs(fd)
+
+
+@pytest.mark.parametrize("physics", PHYS)
+def test_dirichlet_faces_reproduce_periodic_rhs(physics):
+    """Dirichlet faces (EtoB == 1; the reference leaves the branch as
+    "ub = function of x,y,z,t") take the prescribed face state ub.  Cut the
+    periodic box open in x (tag bc1) and prescribe the periodic face average
+    (u- + u+)/2 on the cut faces: with the periodic tau, the RHS is bitwise the
+    periodic one; without the prescription it is not."""
+    per = cube_problem(3, 3)
+    cut = cube_problem(3, 3, bc={"x": "bc1", "y": "pbc", "z": "pbc"})
+    assert (cut.arrays["EtoB"] == 1).sum() == 2 * 3 * 3 * 2
+    u = noisy_state(per, physics, seed=7, eps=0.05)   # traces jump at every face
+    nv = u.shape[0]
+    o_per = O.Oracle(per, physics)
+    r_per, _, tau_per = o_per.rhs(u)
+    vmM = per.mesh.vmapM.ravel()
+    g = 0.5 * (u[:, vmM] + u[:, per.arrays["vmapP"].ravel()])
+    o_cut = O.Oracle(cut, physics)
+    r_open = O.rhs_tau(o_cut, u, tau_per)
+    assert np.abs(r_open - r_per).max() > 1e-6
+    o_cut.set_dirichlet(g)
+    np.testing.assert_array_equal(O.rhs_tau(o_cut, u, tau_per), r_per)
+    assert nv == (5 if physics == "euler" else 1)
+
+
+@pytest.mark.parametrize("physics", PHYS)
+def test_dirichlet_free_stream(physics):
+    """A free stream prescribed on Dirichlet faces (x and z cut open) stays a
+    free stream: zero RHS."""
+    pb = cube_problem(3, 3, bc={"x": "bc1", "y": "pbc", "z": "bc1"})
+    KNp, KNF = pb.K * pb.Np, pb.K * 4 * pb.Nfp
+    if physics == "euler":
+        v = (0.3, -0.2, 0.1)
+        s = np.array([1.0, *v, 1.0 / 0.4 + 0.5 * sum(x * x for x in v)])
+    else:
+        s = np.array([0.7])
+    u = np.repeat(s[:, None], KNp, axis=1)
+    o = O.Oracle(pb, physics)
+    o.set_dirichlet(np.repeat(s[:, None], KNF, axis=1))
+    r, _, _ = o.rhs(u)
+    assert np.abs(r).max() < 1e-12
