@@ -110,14 +110,14 @@ def profiled_traffic(ksym, args):
     factors from a plain copy of known size), when the summary covers this
     kernel and configuration; (None, None) otherwise."""
     path = os.path.join("profiles", f"stage_ws_kernel_euler_N{args.order}_K{6 * args.nhex ** 3}.txt")
-    if ksym != f"stage_ws_kernel<{args.order}, 0>" or args.precision != "fp64" or \
+    if ksym != f"stage_ws_kernel<{args.order}, 0, false>" or args.precision != "fp64" or \
             not os.path.exists(os.path.join(ROOT, path)):
         return None, None
     in_table = False  # the memory-counter table: ... disp, FETCH KiB, WRITE KiB, HBM MB
     for line in open(os.path.join(ROOT, path)):
         if "FETCH KiB" in line and "WRITE KiB" in line:
             in_table = True
-        elif in_table and line.startswith(f"void dglm::stage_ws_kernel<{args.order}, 0>"):
+        elif in_table and line.startswith(f"void dglm::stage_ws_kernel<{args.order}, 0, false>"):
             f = line.split()
             try:
                 fetch, write = float(f[-3]), float(f[-2])
@@ -270,7 +270,7 @@ def main():
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     else:
         roof = {"bound": "mfma", "achieved": tfl, "peak": pk, "unit": "TFLOP/s", "frac": tfl / pk}
-    ksym = f"stage_ws_kernel<{args.order}, 0>" if kernel_name == "mfma_ws" else \
+    ksym = f"stage_ws_kernel<{args.order}, 0, false>" if kernel_name == "mfma_ws" else \
         f"stage_{kernel_name}_kernel<{args.order}, EULER, 0>"
     roof["traffic"], roof["traffic_source"] = profiled_traffic(ksym, args)
     roof["kernel"] = ksym

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HOST_SRCS = ["host/refelem.cpp", "host/mesh.cpp"]
-HIP_SRCS = ["solver.hip"]
+HIP_SRCS = ["solver.hip", "ws_dirichlet.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -4,7 +4,7 @@
 //
 // Math follows the reference CPU path:
 //   calc_maxv   dim3/src/cpp/euler_dglm.cpp:6-50,   advec_dglm.cpp:6-35
-//   calc_dt     euler_dglm.cpp:53-69
+//   calc_dt     euler_dglm.cpp:53-69 (its kernel is in dg_step.hpp)
 //   update_tau  euler_dglm.cpp:72-84
 //   update_ub + calc_fluxLM  euler_dglm.cpp:87-183, advec_dglm.cpp:69-133
 //   calc_fluxes euler_dglm.cpp:186-251, advec_dglm.cpp:136-169
@@ -155,91 +155,6 @@ __global__ __launch_bounds__(256) void wavespeed_kernel(int K, const double *__r
   }
 }
 
-// dt = CFL/(2N+1) / max(lambda*Fscale), clipped so the run ends exactly at
-// tmax (euler_time.cpp:49-57).  One block.
-__global__ __launch_bounds__(1024) void finalize_dt_kernel(const double *__restrict__ partial,
-                                                            const int *__restrict__ partial_idx, int n,
-                                                            DevScalars *s, double cfl_fac, double tmax,
-                                                            double *dt_hist, int *fstar_hist, int *clip_hist,
-                                                            int hist_idx) {
-  __shared__ double red[1024];
-  __shared__ int red_i[1024];
-  double m = 0.0;
-  int mi = 0x7fffffff;
-  for (int i = threadIdx.x; i < n; i += 1024) {
-    const double o = partial[i];
-    if (o > m || (o == m && partial_idx[i] < mi)) {
-      m = o;
-      mi = partial_idx[i];
-    }
-  }
-  red[threadIdx.x] = m;
-  red_i[threadIdx.x] = mi;
-  __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      const double o = red[threadIdx.x + w];
-      if (o > red[threadIdx.x] || (o == red[threadIdx.x] && red_i[threadIdx.x + w] < red_i[threadIdx.x])) {
-        red[threadIdx.x] = o;
-        red_i[threadIdx.x] = red_i[threadIdx.x + w];
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double maxfs = red[0];
-    const double t = s->t_next;
-    s->maxfs = maxfs;
-    s->fstar = red_i[0];
-    s->clipped = 0;
-    if (t < tmax) {
-      double dt = cfl_fac * (1.0 / maxfs);
-      if (t + dt > tmax) {
-        dt = tmax - t;
-        s->clipped = 1;
-      }
-      s->t = t;
-      s->dt = dt;
-      s->t_next = t + dt;
-      s->tstep += 1;
-      s->active = 1;
-      if (!(dt > 0.0) || !isfinite(dt)) s->bad = 1;
-    } else {
-      s->active = 0;
-      s->dt = 0.0;
-    }
-    if (dt_hist) dt_hist[hist_idx] = s->active ? s->dt : 0.0;
-    if (fstar_hist) fstar_hist[hist_idx] = s->fstar;
-    if (clip_hist) clip_hist[hist_idx] = s->clipped;
-  }
-}
-
-// Conserved integrals sum_e J_e w^T u_e (w = column sums of M), FP64: one
-// thread per element, a fixed-order block reduction, partials per block.
-__global__ __launch_bounds__(256) void integrals_kernel(int K, int Np, int nvar, const double *__restrict__ u,
-                                                       const double *__restrict__ J,
-                                                       const double *__restrict__ w, double *partial) {
-  __shared__ double red[256];
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  const size_t KNp = (size_t)K * Np;
-  for (int v = 0; v < nvar; v++) {
-    double s = 0.0;
-    if (e < K) {
-      const double *ue = u + v * KNp + (size_t)e * Np;
-      for (int i = 0; i < Np; i++) s += w[i] * ue[i];
-      s *= J[e];
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int q = 128; q > 0; q >>= 1) {
-      if (threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * nvar + v] = red[0];
-    __syncthreads();
-  }
-}
-
 // --------------------------------------------------------------------------
 // Fused stage kernel, straightforward LDS-staged form.
 //   MODE 0: k = a k + dt rhs; u_out = u_in + b k
@@ -261,7 +176,21 @@ struct StageArgs {
   double rka, rkb;
   double gamma, tau_scale;
   int flux;  // 0: LM (reference), 1: Rusanov (upwind for advection)
+  // Dirichlet faces (EtoB == 1): dmap[e*4+f] = slot of the face (-1: not
+  // Dirichlet), gbnd[v][slot][j] = prescribed state at its face node j (Fmask
+  // order); nd slots.  dmap null: the mesh has none.
+  const int *dmap;
+  const double *gbnd;
+  int nd;
 };
+
+// Prescribed boundary state of variable v at face node j of Dirichlet slot ds.
+// The LM flux takes it as the face state ub (the reference leaves the branch
+// as "ub = function of x,y,z,t", euler_dglm.cpp:136-141); the Rusanov option
+// takes it as the exterior trace u+.
+__device__ inline double dirichlet_value(const StageArgs &a, int v, int ds, int j, int Nfp) {
+  return a.gbnd[((size_t)v * a.nd + ds) * Nfp + j];
+}
 
 template <int N, int PHYS>
 constexpr int eb_simple() {
@@ -286,6 +215,7 @@ __global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
   __shared__ double s_geo[EB * GEO_STRIDE];
   __shared__ double s_tau[EB * 4];
   __shared__ int s_conn[EB * 4];
+  __shared__ int s_dm[EB * 4];
   __shared__ int s_fmx[16 * Nfp];
 
   const int K = a.K;
@@ -308,6 +238,7 @@ __global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
     s_conn[i] = c;
     const int nf = conn_nbr(c) * 4 + conn_face(c);
     s_tau[i] = 0.25 * (a.maxv[(size_t)e0 * 4 + i] + a.maxv[nf]) * a.tau_scale;
+    s_dm[i] = a.dmap ? a.dmap[(size_t)e0 * 4 + i] : -1;
   }
   for (int v = 0; v < NV; v++)
     for (int i = tid; i < ne * Np; i += 256) s_u[v][i] = a.u_in[v * KNp + (size_t)e0 * Np + i];
@@ -340,7 +271,8 @@ __global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
     }
   }
 
-  // LM surface flux (n.F(ub) - tau (ub - u-)) Fscale with ub = (u- + u+)/2.
+  // LM surface flux (n.F(ub) - tau (ub - u-)) Fscale with ub = (u- + u+)/2,
+  // or ub = the prescribed state on Dirichlet faces (Rusanov: u+ = it).
   for (int idx = tid; idx < ne * NFF; idx += 256) {
     const int e = idx / NFF, m = idx - e * NFF;
     const int f = m / Nfp, j = m - f * Nfp;
@@ -350,12 +282,13 @@ __global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
     const double *g = &s_geo[e * GEO_STRIDE + 9 + f * 4];
     const double nx = g[0], ny = g[1], nz = g[2], fs = g[3];
     const double tau = s_tau[e * 4 + f];
+    const int ds = s_dm[e * 4 + f];
     if (PHYS == PHYS_EULER) {
       double uM[5], uP[5], ub[5], F[5], G[5], H[5];
       for (int v = 0; v < 5; v++) {
         uM[v] = s_u[v][mloc];
-        uP[v] = a.u_in[v * KNp + pidx];
-        ub[v] = 0.5 * (uM[v] + uP[v]);
+        uP[v] = ds >= 0 ? dirichlet_value(a, v, ds, j, Nfp) : a.u_in[v * KNp + pidx];
+        ub[v] = ds >= 0 ? uP[v] : 0.5 * (uM[v] + uP[v]);
       }
       if (a.flux == 1) {
         for (int v = 0; v < 5; v++) s_f[v][e * KD + 3 * Np + m] = euler_rusanov(v, uM, uP, g, a.gamma) * fs;
@@ -366,8 +299,8 @@ __global__ __launch_bounds__(256) void stage_simple_kernel(StageArgs a) {
       }
     } else {
       const double uM = s_u[0][mloc];
-      const double uP = a.u_in[pidx];
-      const double ub = 0.5 * (uM + uP);
+      const double uP = ds >= 0 ? dirichlet_value(a, 0, ds, j, Nfp) : a.u_in[pidx];
+      const double ub = ds >= 0 && a.flux != 1 ? uP : 0.5 * (uM + uP);
       const double vn = nx * s_vel[0][mloc] + ny * s_vel[1][mloc] + nz * s_vel[2][mloc];
       if (a.flux == 1) {
         s_f[0][e * KD + 3 * Np + m] = (vn * ub + 0.5 * fabs(vn) * (uM - uP)) * fs;

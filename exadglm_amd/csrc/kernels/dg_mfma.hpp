@@ -255,6 +255,7 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
   // fit = [fitV (KD x 4) | fitP (KD x 4) | fitAV (Np x NBLK x 4)]
   const double *fitV = fit, *fitP = fit + 4 * KD, *fitAV = fit + 8 * KD;
   __shared__ int s_conn[EB * 4];
+  __shared__ int s_dm[EB * 4];  // Dirichlet slot of each face (-1: none)
   __shared__ int s_fmx[16 * Nfp];
 
   const int tid = threadIdx.x;
@@ -287,6 +288,7 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
     const int mloc = e * Np + s_fmx[(f * 4 + 3) * Nfp + j];
     const double *g = &s_geo[e * GEO_STRIDE + 9 + f * 4];
     const double tau = s_tau[e * 4 + f];
+    const int ds = s_dm[e * 4 + f];
     if (PHYS == PHYS_EULER) {
       double ub[5];
       if (a.flux == 1) {
@@ -294,12 +296,13 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
 #pragma unroll
         for (int q = 0; q < 5; q++) {
           uM[q] = s_u[q][mloc];
-          uP[q] = a.u_in[q * KNp + pidx];
+          uP[q] = ds >= 0 ? dirichlet_value(a, q, ds, j, Nfp) : a.u_in[q * KNp + pidx];
         }
         return euler_rusanov(v, uM, uP, g, a.gamma) * g[3];
       }
 #pragma unroll
-      for (int q = 0; q < 5; q++) ub[q] = 0.5 * (s_u[q][mloc] + a.u_in[q * KNp + pidx]);
+      for (int q = 0; q < 5; q++)
+        ub[q] = ds >= 0 ? dirichlet_value(a, q, ds, j, Nfp) : 0.5 * (s_u[q][mloc] + a.u_in[q * KNp + pidx]);
       const double inv = 1.0 / ub[0];
       const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
       const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;  // n . velocity
@@ -310,8 +313,8 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
       return (nF - tau * (ub[v] - s_u[v][mloc])) * g[3];
     } else {
       const double uM = s_u[0][mloc];
-      const double uP = a.u_in[pidx];
-      const double ub = 0.5 * (uM + uP);
+      const double uP = ds >= 0 ? dirichlet_value(a, 0, ds, j, Nfp) : a.u_in[pidx];
+      const double ub = ds >= 0 && a.flux != 1 ? uP : 0.5 * (uM + uP);
       if (a.flux == 1) {
         const double vn = g[0] * s_vel[0][mloc] + g[1] * s_vel[1][mloc] + g[2] * s_vel[2][mloc];
         return (vn * ub + 0.5 * fabs(vn) * (uM - uP)) * g[3];
@@ -330,6 +333,7 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
       const int c = a.conn[(size_t)e0 * 4 + i];
       s_conn[i] = c;
       s_tau[i] = 0.25 * (a.maxv[(size_t)e0 * 4 + i] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) * a.tau_scale;
+      s_dm[i] = a.dmap ? a.dmap[(size_t)e0 * 4 + i] : -1;
     }
 #pragma unroll
     for (int v = 0; v < NV; v++)

@@ -165,9 +165,19 @@ __device__ inline int ws_role(int hw) {
 
 __device__ inline int ws_batch(const int *order, int b) { return order ? order[b] : b; }
 
+// Dirichlet flags of a producer lane's pairs: bit s set when pair s lies on
+// a Dirichlet face (up[s] then holds the prescribed state).  Empty without
+// Dirichlet faces, so that instantiation's register state is unchanged.
+template <bool DIR>
+struct WsDirBits {
+  unsigned dm;
+};
+template <>
+struct WsDirBits<false> {};
+
 // Registers a producer wave carries from one staging step to the next.
-template <int N>
-struct WsProd {
+template <int N, bool DIR>
+struct WsProd : WsDirBits<DIR> {
   static constexpr int NRS = (WsShape<N>::PPW + 63) / 64;
   double up[NRS][5];        // exterior traces of this lane's face-node pairs
   double lam_own, lam_nbr;  // wave speeds of face `lane` of the batch (producer 0)
@@ -187,13 +197,14 @@ struct WsProd {
 //   ws_land    (phase 1) wait for all of it; producer 0 writes tau and the
 //              next connectivity.
 // The other producers, and the consumers, read what one producer wrote only
-// after the barrier that ends the phase.
-template <int N>
+// after the barrier that ends the phase.  DIR: the mesh has Dirichlet faces;
+// their pairs load the prescribed state instead of the exterior trace.
+template <int N, bool DIR>
 __device__ inline void ws_gather(const StageArgs &a, const int *s_fmx, const int *cb_cur, int b, int nb, int pw,
-                                 int lane_in, WsProd<N> &pr) {
+                                 int lane_in, WsProd<N, DIR> &pr) {
   using S = WsShape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4, PPW = S::PPW;
-  constexpr int NRS = WsProd<N>::NRS;
+  constexpr int NRS = WsProd<N, DIR>::NRS;
   // lane-derived indices are the same every call: an opaque copy of the lane
   // id keeps them recomputed here rather than held in registers across the
   // batch loop
@@ -218,6 +229,7 @@ __device__ inline void ws_gather(const StageArgs &a, const int *s_fmx, const int
   const double *ubase[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) ubase[v] = a.u_in + v * KNp;
+  if constexpr (DIR) pr.dm = 0;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
     const int pl = lane + 64 * s, P = PPW * pw + pl;
@@ -225,13 +237,29 @@ __device__ inline void ws_gather(const StageArgs &a, const int *s_fmx, const int
     // dead pairs load node 0 (never used): no select on the loaded values,
     // so nothing here waits for the loads
     int pidx = 0;  // K*Np < 2^31 (checked on the host)
+    int ds = -1, jd = 0;
     if (pl < PPW && P < S::NPAIR && q < ne && m < NFF) {
       const int f = m / Nfp, jf = m - f * Nfp;
       const int cc = cb_cur[q * 4 + f];
       pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
+      if constexpr (DIR) {
+        ds = a.dmap[(size_t)(e0 + q) * 4 + f];
+        jd = jf;
+      }
     }
+    if constexpr (DIR) {
+      if (ds >= 0) {
+        pr.dm |= 1u << s;
 #pragma unroll
-    for (int v = 0; v < 5; v++) pr.up[s][v] = ubase[v][pidx];
+        for (int v = 0; v < 5; v++) pr.up[s][v] = dirichlet_value(a, v, ds, jd, Nfp);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 5; v++) pr.up[s][v] = ubase[v][pidx];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 5; v++) pr.up[s][v] = ubase[v][pidx];
+    }
   }
 }
 
@@ -259,9 +287,9 @@ __device__ inline void ws_dma(const StageArgs &a, WsSlot<N> &sl, int b, int pw, 
   }
 }
 
-template <int N>
+template <int N, bool DIR>
 __device__ inline void ws_land(const StageArgs &a, WsSlot<N> &sl, int *cb_next, int pw, int lane,
-                               const WsProd<N> &pr) {
+                               const WsProd<N, DIR> &pr) {
   // ---- every load and DMA copy of this wave has landed ----
   __builtin_amdgcn_s_waitcnt(0);
   if (pw == 0) {
@@ -290,13 +318,14 @@ __device__ inline void ws_primitives(const StageArgs &a, WsSlot<N> &sl, int t, i
 
 // LM (or Rusanov) surface flux of the five variables times Fscale at this
 // producer's face-node pairs of batch b, into fs ([variable][face node]
-// [element], row stride FS); padding entries become zero.
-template <int N>
+// [element], row stride FS); padding entries become zero.  On a Dirichlet
+// pair the LM face state ub is the prescribed state (Rusanov: it is u+).
+template <int N, bool DIR>
 __device__ inline void ws_surface(const StageArgs &a, const WsSlot<N> &sl, double *fs, const int *s_fmx, int b,
-                                  int pw, int lane_in, const WsProd<N> &pr) {
+                                  int pw, int lane_in, const WsProd<N, DIR> &pr) {
   using S = WsShape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4, PPW = S::PPW;
-  constexpr int NRS = WsProd<N>::NRS;
+  constexpr int NRS = WsProd<N, DIR>::NRS;
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
   const int ne = min(EB, a.K - b * EB);
@@ -322,6 +351,12 @@ __device__ inline void ws_surface(const StageArgs &a, const WsSlot<N> &sl, doubl
           double ub[5];
 #pragma unroll
           for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + pr.up[s][v]);
+          if constexpr (DIR) {
+            if ((pr.dm >> s) & 1u) {
+#pragma unroll
+              for (int v = 0; v < 5; v++) ub[v] = pr.up[s][v];
+            }
+          }
           const double inv = 1.0 / ub[0];
           const double p = (a.gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
           const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
@@ -511,11 +546,13 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
 }
 
 //   MODE 0: LSRK update; MODE 1: write the RHS only.
+//   DIR: the mesh has Dirichlet faces (a.dmap / a.gbnd are read); the
+//   instantiation without them carries none of that code.
 // ngrp: number of XCD groups the grid is split into (8, or 1 for plain
 // striding); gridDim.x must be a multiple of ngrp.
 // order: batch visiting order (physical batch order[b] for schedule position b;
 // null: b itself).
-template <int N, int MODE>
+template <int N, int MODE, bool DIR>
 __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, const double *__restrict__ afrag,
                                                                    const double *__restrict__ arem,
                                                                    const int *__restrict__ order, int nbatch,
@@ -570,31 +607,31 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
     const int pw = w - NC;
     // physical batch of schedule position x (-1: past the range)
     auto pb = [&](int x) { return x < hi ? ws_batch(order, x) : -1; };
-    WsProd<N> pr;
+    WsProd<N, DIR> pr;
     if (b < hi) {
-      ws_gather<N>(a, s_fmx, s_cb[0], pb(b), pb(b + gsz), pw, lane, pr);
+      ws_gather<N, DIR>(a, s_fmx, s_cb[0], pb(b), pb(b + gsz), pw, lane, pr);
       ws_dma<N>(a, sl[0], pb(b), pw, lane);
-      ws_land<N>(a, sl[0], s_cb[1], pw, lane, pr);
+      ws_land<N, DIR>(a, sl[0], s_cb[1], pw, lane, pr);
     }
     __syncthreads();  // batch b staged
     if (b < hi) {
       ws_primitives<N>(a, sl[0], pw * 64 + lane, S::NPW * 64);
-      ws_surface<N>(a, sl[0], s_fs, s_fmx, pb(b), pw, lane, pr);
+      ws_surface<N, DIR>(a, sl[0], s_fs, s_fmx, pb(b), pw, lane, pr);
     }
-    if (b + gsz < hi) ws_gather<N>(a, s_fmx, s_cb[1], pb(b + gsz), pb(b + 2 * gsz), pw, lane, pr);
+    if (b + gsz < hi) ws_gather<N, DIR>(a, s_fmx, s_cb[1], pb(b + gsz), pb(b + 2 * gsz), pw, lane, pr);
     __syncthreads();  // batch b's primitives and surface fluxes complete
     for (int it = 0; b < hi; b += gsz, it++) {
       const int nb = b + gsz, nnb = b + 2 * gsz;
       if (nb < hi) {
         ws_dma<N>(a, sl[(it + 1) & 1], pb(nb), pw, lane);
-        ws_land<N>(a, sl[(it + 1) & 1], s_cb[it & 1], pw, lane, pr);
+        ws_land<N, DIR>(a, sl[(it + 1) & 1], s_cb[it & 1], pw, lane, pr);
       }
       __syncthreads();  // consumers are done with the surface buffer; batch nb staged
       if (nb < hi) {
         ws_primitives<N>(a, sl[(it + 1) & 1], pw * 64 + lane, S::NPW * 64);
-        ws_surface<N>(a, sl[(it + 1) & 1], s_fs, s_fmx, pb(nb), pw, lane, pr);
+        ws_surface<N, DIR>(a, sl[(it + 1) & 1], s_fs, s_fmx, pb(nb), pw, lane, pr);
       }
-      if (nnb < hi) ws_gather<N>(a, s_fmx, s_cb[it & 1], pb(nnb), pb(nnb + gsz), pw, lane, pr);
+      if (nnb < hi) ws_gather<N, DIR>(a, s_fmx, s_cb[it & 1], pb(nnb), pb(nnb + gsz), pw, lane, pr);
       __syncthreads();
     }
     return;

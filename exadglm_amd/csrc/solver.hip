@@ -21,10 +21,18 @@
 #include "host/mesh.h"
 #include "host/refelem.h"
 #include "kernels/dg_fp64.hpp"
+#include "kernels/dg_step.hpp"
 #include "kernels/dg_mfma.hpp"
 #include "kernels/dg_ws.hpp"
 #include "kernels/dg_adjoint.hpp"
 #include "kernels/dg_adjoint_mfma.hpp"
+
+namespace dglm {
+// ws_dirichlet.hip: the warp-specialised stage kernel on meshes with Dirichlet
+// faces (false: not built for this order).
+bool launch_ws_dirichlet(int N, int mode, int grid, hipStream_t stream, const StageArgs &a, const double *afrag,
+                         const double *arem, const int *order, int nbatch, int groups);
+}  // namespace dglm
 
 #define DGLM_NTIMERS 3  // wave speed, dt finalize, stage
 
@@ -81,6 +89,13 @@ struct dglm_ctx {
   double *d_J = nullptr, *d_w = nullptr, *d_ipart = nullptr;  // conservation sums
   bool adj_exact = true;          // include the tau(u) and dt(u) sensitivities (Euler)
   std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
+  // Dirichlet faces (EtoB == 1): slot map [K*4] (-1: none), prescribed states
+  // [nvar][nd][Nfp] and the faces of the slots in ascending order.
+  int nd = 0;
+  int *dmap = nullptr;
+  double *gbnd = nullptr;
+  bool g_set = false;
+  std::vector<int> h_dface;
   // Optional per-kernel timing with HIP events on the context's stream.
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -270,7 +285,7 @@ void launch_ws(dglm_ctx *c, const StageArgs &a) {
       int dev = 0, cus = 0, occ = 0;
       hipGetDevice(&dev);
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_ws_kernel<N, 0>, S::NT, 0);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_ws_kernel<N, 0, false>, S::NT, 0);
       c->ws_grid = (cus > 0 ? cus : 256) * (occ > 0 ? occ : 1);
       c->ws_groups = c->ws_grid % 8 == 0 ? 8 : 1;
     }
@@ -279,8 +294,11 @@ void launch_ws(dglm_ctx *c, const StageArgs &a) {
       grid = nbatch;
       groups = 1;
     }
-    hipLaunchKernelGGL((stage_ws_kernel<N, MODE>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->afrag_ws,
-                       c->arem_ws, c->ws_order, nbatch, groups);
+    if (a.dmap)
+      launch_ws_dirichlet(N, MODE, grid, c->stream, a, c->afrag_ws, c->arem_ws, c->ws_order, nbatch, groups);
+    else
+      hipLaunchKernelGGL((stage_ws_kernel<N, MODE, false>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->afrag_ws,
+                         c->arem_ws, c->ws_order, nbatch, groups);
   }
 }
 
@@ -373,6 +391,9 @@ StageArgs stage_args(dglm_ctx *c, int s) {
   a.gamma = c->p.gamma;
   a.tau_scale = c->p.tau_scale;
   a.flux = c->p.flux == DGLM_FLUX_RUSANOV ? 1 : 0;
+  a.dmap = c->nd > 0 ? c->dmap : nullptr;
+  a.gbnd = c->gbnd;
+  a.nd = c->nd;
   return a;
 }
 
@@ -391,7 +412,7 @@ void free_ctx(dglm_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
                   c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx, c->afrag_ws, c->arem_ws,
-                  c->ws_order};
+                  c->ws_order, c->dmap, c->gbnd};
   for (void *q : ptrs)
     if (q) hipFree(q);
   free_tape(c->tape);
@@ -485,7 +506,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
 
   // Packed connectivity from (vmapF, vmapP): find the neighbour face and the
   // node permutation that reproduces vmapP exactly.
-  std::vector<int> conn((size_t)K * 4);
+  std::vector<int> conn((size_t)K * 4), dmap((size_t)K * 4, -1), dface;
   for (int e = 0; e < K; e++)
     for (int f = 0; f < 4; f++) {
       const int o = e * 4 + f;
@@ -501,8 +522,13 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       if (found < 0)
         return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: vmapP of face " + std::to_string(o) +
                                               " is not a rotation of its neighbour face");
-      if (m->EtoB && m->EtoB[o] == 1 && found != 3)
-        return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: Dirichlet faces (EtoB == 1) are not implemented");
+      if (m->EtoB && m->EtoB[o] == 1) {
+        if (found != 3 || nb != e || nf != f)
+          return fail(DGLM_ERR_ARG, "dglm_create: Dirichlet face " + std::to_string(o) +
+                                        " (EtoB == 1) is not a boundary face (vmapP must be vmapM)");
+        dmap[o] = (int)dface.size();
+        dface.push_back(o);
+      }
       conn[o] = (nb << 4) | (nf << 2) | found;
     }
 
@@ -622,6 +648,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   c->dev = p->device;
   c->nblk_ws = (K + wavespeed_elems(Np) - 1) / wavespeed_elems(Np);
   c->h_conn = conn;
+  c->nd = (int)dface.size();
+  c->h_dface = dface;
   int rc;
   hipError_t he = hipSetDevice(c->dev);
   if (he != hipSuccess) {
@@ -648,6 +676,11 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
     free_ctx(c);
     return rc;
   }
+  if (c->nd > 0 && ((rc = dev_alloc(&c->dmap, dmap.size(), "Dirichlet slot map")) ||
+                    (rc = dev_alloc(&c->gbnd, (size_t)c->nvar * c->nd * Nfp, "Dirichlet states")))) {
+    free_ctx(c);
+    return rc;
+  }
   he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (he != hipSuccess) {
     free_ctx(c);
@@ -669,6 +702,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
                   : hipSuccess,
       c->rowsum ? hipMemcpy(c->rowsum, rsum.data(), rsum.size() * sizeof(double), hipMemcpyHostToDevice)
                 : hipSuccess,
+      c->dmap ? hipMemcpy(c->dmap, dmap.data(), dmap.size() * sizeof(int), hipMemcpyHostToDevice) : hipSuccess,
       hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
       hipMemset(c->u[0], 0, nst * sizeof(double)),
       hipMemset(c->u[1], 0, nst * sizeof(double)),
@@ -815,6 +849,40 @@ extern "C" int dglm_set_velocity(dglm_ctx *c, const double *vx, const double *vy
   return DGLM_OK;
 }
 
+extern "C" int dglm_dirichlet_faces(const dglm_ctx *c, int *nd, int *faces) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_dirichlet_faces: null context");
+  if (nd) *nd = c->nd;
+  if (faces && c->nd > 0) memcpy(faces, c->h_dface.data(), c->h_dface.size() * sizeof(int));
+  return DGLM_OK;
+}
+
+static int set_dirichlet_impl(dglm_ctx *c, const double *g, hipMemcpyKind kind) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_set_dirichlet: null context");
+  if (c->nd == 0) return fail(DGLM_ERR_ARG, "dglm_set_dirichlet: the mesh has no Dirichlet faces (EtoB == 1)");
+  if (!g) return fail(DGLM_ERR_ARG, "dglm_set_dirichlet: null states");
+  HIP_TRY(hipSetDevice(c->dev));
+  const size_t n = (size_t)c->nvar * c->nd * c->Nfp;
+  HIP_TRY(hipMemcpyAsync(c->gbnd, g, n * sizeof(double), kind, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->g_set = true;
+  return DGLM_OK;
+}
+
+extern "C" int dglm_set_dirichlet(dglm_ctx *c, const double *g) {
+  return set_dirichlet_impl(c, g, hipMemcpyHostToDevice);
+}
+extern "C" int dglm_set_dirichlet_device(dglm_ctx *c, const double *d_g) {
+  return set_dirichlet_impl(c, d_g, hipMemcpyDeviceToDevice);
+}
+
+// Stepping a mesh with Dirichlet faces needs their states.
+static int need_dirichlet(const dglm_ctx *c, const char *who) {
+  if (c->nd > 0 && !c->g_set)
+    return fail(DGLM_ERR_ARG, std::string(who) + ": the mesh has Dirichlet faces (EtoB == 1) and no prescribed "
+                                                 "states (dglm_set_dirichlet)");
+  return DGLM_OK;
+}
+
 extern "C" int dglm_set_time(dglm_ctx *c, double t, int tstep) {
   if (!c) return fail(DGLM_ERR_ARG, "dglm_set_time: null context");
   HIP_TRY(hipSetDevice(c->dev));
@@ -844,6 +912,7 @@ extern "C" int dglm_get_time(dglm_ctx *c, double *t, double *dt, int *tstep, int
 
 extern "C" int dglm_rhs(dglm_ctx *c, const double *u, double *rhs, double *maxv, double *tau) {
   if (!c || !u || !rhs) return fail(DGLM_ERR_ARG, "dglm_rhs: null argument");
+  if (int rc = need_dirichlet(c, "dglm_rhs")) return rc;
   ws_fn ws;
   st_fn st0, st1;
   if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_rhs: order not built");
@@ -925,6 +994,7 @@ int build_graph(dglm_ctx *c, ws_fn ws, st_fn st0) {
 
 extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
   if (!c || nsteps < 0) return fail(DGLM_ERR_ARG, "dglm_step: bad arguments");
+  if (int rc = need_dirichlet(c, "dglm_step")) return rc;
   ws_fn ws;
   st_fn st0, st1;
   if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_step: order not built");
@@ -1301,6 +1371,7 @@ AdjArgs adj_args(dglm_ctx *c) {
 extern "C" int dglm_adjoint_rhs(dglm_ctx *c, const double *u, const double *rbar, double *ubar, double *taubar) {
   if (!c || !u || !rbar || !ubar) return fail(DGLM_ERR_ARG, "dglm_adjoint_rhs: null argument");
   if (c->p.flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: built for the LM flux only");
+  if (c->nd > 0) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: not built for meshes with Dirichlet faces");
   HIP_TRY(hipSetDevice(c->dev));
   int rc = ensure_adjoint_buffers(c);
   if (rc) return rc;
@@ -1336,6 +1407,7 @@ extern "C" int dglm_adjoint_rhs(dglm_ctx *c, const double *u, const double *rbar
 extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
   if (!c || nsteps <= 0 || every <= 0) return fail(DGLM_ERR_ARG, "dglm_adjoint_record: bad arguments");
   if (c->p.flux != DGLM_FLUX_LM) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: built for the LM flux only");
+  if (c->nd > 0) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: not built for meshes with Dirichlet faces");
   HIP_TRY(hipSetDevice(c->dev));
   int rc = ensure_adjoint_buffers(c);
   if (rc) return rc;
@@ -1392,6 +1464,7 @@ extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *
   if (!c || !ubar_T || !ubar_0) return fail(DGLM_ERR_ARG, "dglm_adjoint_backward: null argument");
   dglm_adj_tape *T = c->tape;
   if (!T || T->nsteps == 0) return fail(DGLM_ERR_STATE, "dglm_adjoint_backward: nothing recorded");
+  if (c->nd > 0) return fail(DGLM_ERR_UNSUPPORTED, "adjoint: not built for meshes with Dirichlet faces");
   HIP_TRY(hipSetDevice(c->dev));
   ws_fn ws;
   st_fn st0, st1;

@@ -102,6 +102,34 @@ class Solver:
         v = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (n,))) for a in (vx, vy, vz)]
         A.check(A.lib().dglm_set_velocity(self._ctx, A.dptr(v[0]), A.dptr(v[1]), A.dptr(v[2])))
 
+    def dirichlet_faces(self):
+        """Face indices e*4 + f of the Dirichlet faces (EtoB == 1) in
+        ascending order: the slot order of set_dirichlet."""
+        nd = C.c_int()
+        A.check(A.lib().dglm_dirichlet_faces(self._ctx, C.byref(nd), None))
+        faces = np.empty(nd.value, np.int32)
+        if nd.value:
+            A.check(A.lib().dglm_dirichlet_faces(self._ctx, C.byref(nd), A.iptr(faces)))
+        return faces
+
+    def set_dirichlet(self, g):
+        """Prescribed boundary states on the Dirichlet faces: g [nvar][nd][Nfp]
+        (slots as dirichlet_faces()), or a face-node array [nvar][K*4*Nfp]
+        (vmapM order) whose Dirichlet entries are taken.  LM flux: the face
+        state ub; Rusanov: the exterior trace u+."""
+        g = np.asarray(g, dtype=np.float64)
+        faces = self.dirichlet_faces()
+        nd, nfp = len(faces), self.Nfp
+        if g.size == self.nvar * self.K * 4 * nfp and g.size != self.nvar * nd * nfp:
+            g = g.reshape(self.nvar, self.K * 4, nfp)[:, faces, :]
+        if g.size != self.nvar * nd * nfp:
+            raise ValueError(f"Dirichlet states must have {self.nvar}x{nd}x{nfp} entries, got {g.shape}")
+        g = np.ascontiguousarray(g.reshape(self.nvar, nd, nfp))
+        A.check(A.lib().dglm_set_dirichlet(self._ctx, A.dptr(g)))
+
+    def set_dirichlet_device(self, ptr):
+        A.check(A.lib().dglm_set_dirichlet_device(self._ctx, C.cast(C.c_void_p(ptr), A.PD)))
+
     def set_time(self, t=0.0, tstep=0):
         A.check(A.lib().dglm_set_time(self._ctx, C.c_double(t), tstep))
 

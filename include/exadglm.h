@@ -147,6 +147,27 @@ int dglm_set_state_device(dglm_ctx *ctx, const double *d_u);
 int dglm_get_state_device(dglm_ctx *ctx, double *d_u);
 /* Advection velocity per node (vx, vy, vz each [K*Np]); advec_init.cpp:5-23. */
 int dglm_set_velocity(dglm_ctx *ctx, const double *vx, const double *vy, const double *vz);
+
+/* Dirichlet faces: the faces with EtoB == 1, which must be boundary faces
+ * (vmapP = vmapM there).  The reference's calc_fluxLM has the branch as a
+ * stub, "ub = function of x,y,z,t" (euler_dglm.cpp:136-141,
+ * advec_dglm.cpp:113-119), and leaves ub unset; here the caller prescribes
+ * the boundary state g per face node.  With the LM flux it is the face state
+ * ub (the tau term is tau (g - u-)); with the Rusanov option it is the
+ * exterior trace u+.
+ *
+ * dglm_dirichlet_faces: nd = their number; faces (optional, [nd]) = their
+ * face indices e*4 + f in ascending order, the slot order of g.
+ * dglm_set_dirichlet: g host [nvar][nd][Nfp], node j of a face in the order
+ * of its vmapM entries (Fmask[f]).  The states are held until set again:
+ * constant over the stages and steps of a dglm_step call (step one at a time
+ * and set g between calls for time-dependent data).  dglm_step / dglm_rhs on
+ * a mesh with Dirichlet faces return DGLM_ERR_ARG until g has been set once.
+ * The adjoint refuses such meshes (DGLM_ERR_UNSUPPORTED). */
+int dglm_dirichlet_faces(const dglm_ctx *ctx, int *nd, int *faces);
+int dglm_set_dirichlet(dglm_ctx *ctx, const double *g);
+int dglm_set_dirichlet_device(dglm_ctx *ctx, const double *d_g);
+
 int dglm_set_time(dglm_ctx *ctx, double t, int tstep);
 /* done != 0 once t has reached tmax. */
 int dglm_get_time(dglm_ctx *ctx, double *t, double *dt, int *tstep, int *done);

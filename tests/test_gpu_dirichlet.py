@@ -1,0 +1,162 @@
+"""Dirichlet faces (EtoB == 1) on the HIP path, against the oracle.
+
+The reference leaves the Dirichlet branch of calc_fluxLM as a stub ("ub =
+function of x,y,z,t", euler_dglm.cpp:136-141); here the caller prescribes the
+boundary state g per face node: the LM flux takes it as ub, the Rusanov
+option as the exterior trace u+.  The oracle's Dirichlet branch is pinned on
+the CPU (a cut box with the periodic face average prescribed reproduces the
+periodic RHS bitwise; free streams stay free; finite differences of the
+tangent-linear); these tests hold every stage kernel to it, on a box cut open
+in x and z (tag bc1) and periodic in y."""
+import numpy as np
+import pytest
+
+from helpers import noisy_state, rel_err, rel_err_per_var, smooth_state
+from exadglm_amd.problem import cube_problem
+from exadglm_amd.solver import Solver
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+PHYS = ["advection", "euler"]
+CUT = {"x": "bc1", "y": "pbc", "z": "bc1"}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _variants(N, physics):
+    return ["simple"] + (["mfma"] if N <= 6 else []) + (["mfma_ws"] if physics == "euler" and N in (3, 4) else [])
+
+
+def _face_states(pb, u, seed):
+    """A prescribed state on every face node ([nvar][K*4*Nfp], vmapM order):
+    the interior trace with 5% relative noise (physical for Euler)."""
+    rng = np.random.default_rng(seed)
+    g = u[:, pb.mesh.vmapM.ravel()]
+    return g * (1 + 0.05 * rng.standard_normal(g.shape))
+
+
+@pytest.mark.parametrize("physics", PHYS)
+@pytest.mark.parametrize("N,variant", [(3, "simple"), (3, "mfma"), (3, "mfma_ws"), (4, "mfma"), (4, "mfma_ws"),
+                                       (7, "simple")])
+def test_dirichlet_parity(gpu, physics, N, variant):
+    """LM flux: RHS (with wave speeds and tau) and 3 LSRK steps against the
+    oracle with the same prescribed states."""
+    if variant not in _variants(N, physics):
+        pytest.skip(f"no {variant} kernel for {physics} N={N}")
+    _say(f"dirichlet parity {physics} N={N} {variant}")
+    pb = cube_problem(N, 3, bc=CUT)
+    assert (pb.arrays["EtoB"] == 1).sum() == 2 * 2 * 9 * 2
+    u = noisy_state(pb, physics, seed=N + 3, eps=0.02)
+    g = _face_states(pb, u, seed=N)
+    with Solver(pb, physics, cfl=0.5) as s:
+        s.set_stage_kernel(variant)
+        assert s.stage_kernel() == variant
+        np.testing.assert_array_equal(s.dirichlet_faces(), np.nonzero(pb.arrays["EtoB"].ravel() == 1)[0])
+        s.set_dirichlet(g)
+        r, mv, tau = s.rhs(u)
+        s.set_state(u)
+        s.step(3)
+        u3 = s.get_state()
+    o = O.Oracle(pb, physics, cfl=0.5)
+    o.set_dirichlet(g)
+    ro, mvo, tauo = o.rhs(u)
+    uo3 = o.run(u, 3)[0]
+    assert rel_err(mv, mvo) < 1e-15
+    assert rel_err(tau, tauo) < 1e-15
+    assert max(rel_err_per_var(r, ro)) < 1e-12
+    assert max(rel_err_per_var(u3, uo3)) < 1e-12
+    # the prescription matters: the RHS without it (ub = u- on those faces) differs
+    ro_copy = O.Oracle(pb, physics, cfl=0.5).rhs(u)[0]
+    assert rel_err(ro_copy, ro) > 1e-4
+
+
+@pytest.mark.parametrize("physics", PHYS)
+@pytest.mark.parametrize("N,variant", [(3, "simple"), (3, "mfma"), (4, "mfma_ws")])
+def test_dirichlet_rusanov_reproduces_periodic(gpu, physics, N, variant):
+    """Rusanov option: prescribing the periodic exterior trace u+ on the cut
+    faces gives the RHS of the periodic box (the Rusanov flux has no tau)."""
+    if variant not in _variants(N, physics):
+        pytest.skip(f"no {variant} kernel for {physics} N={N}")
+    _say(f"dirichlet rusanov periodic {physics} N={N} {variant}")
+    per = cube_problem(N, 3)
+    cut = cube_problem(N, 3, bc=CUT)
+    u = noisy_state(per, physics, seed=5, eps=0.02)
+    g = u[:, per.arrays["vmapP"].ravel()]
+    res = {}
+    for name, pb in (("per", per), ("cut", cut)):
+        with Solver(pb, physics, flux="rusanov") as s:
+            s.set_stage_kernel(variant)
+            if name == "cut":
+                s.set_dirichlet(g)
+            res[name] = s.rhs(u)[0]
+    assert max(rel_err_per_var(res["cut"], res["per"])) < 1e-14
+
+
+@pytest.mark.parametrize("flux", ["lm", "rusanov"])
+@pytest.mark.parametrize("physics", PHYS)
+def test_dirichlet_free_stream(gpu, physics, flux):
+    """A free stream prescribed on the Dirichlet faces stays a free stream
+    over 5 steps on the default kernel."""
+    _say(f"dirichlet free stream {physics} {flux}")
+    pb = cube_problem(4, 3, bc=CUT)
+    KNp = pb.K * pb.Np
+    if physics == "euler":
+        v = (0.3, -0.2, 0.1)
+        st = np.array([1.0, *v, 1.0 / 0.4 + 0.5 * sum(x * x for x in v)])
+    else:
+        st = np.array([0.7])
+    u0 = np.repeat(st[:, None], KNp, axis=1)
+    with Solver(pb, physics, cfl=0.5, flux=flux) as s:
+        s.set_dirichlet(np.repeat(st[:, None], pb.K * 4 * pb.Nfp, axis=1))
+        s.set_state(u0)
+        s.step(5)
+        u = s.get_state()
+    assert np.abs(u - u0).max() < 1e-12
+
+
+def test_dirichlet_states_updated_between_steps(gpu):
+    """Time-dependent data: new states set between single steps are used by
+    the next step (oracle: the same sequence)."""
+    _say("dirichlet update between steps")
+    pb = cube_problem(4, 3, bc=CUT)
+    u0 = smooth_state(pb, "euler", seed=9)
+    g1, g2 = _face_states(pb, u0, 1), _face_states(pb, u0, 2)
+    with Solver(pb, "euler", cfl=0.5) as s:
+        s.set_state(u0)
+        s.set_dirichlet(g1)
+        s.step(1)
+        s.set_dirichlet(g2)
+        s.step(1)
+        u = s.get_state()
+    o = O.Oracle(pb, "euler", cfl=0.5)
+    o.set_dirichlet(g1)
+    u1, _, t1 = o.run(u0, 1)
+    o.set_dirichlet(g2)
+    uo = o.run(u1, 1, t=t1)[0]
+    assert max(rel_err_per_var(u, uo)) < 1e-12
+
+
+def test_dirichlet_errors(gpu):
+    """Stepping without prescribed states, states for a mesh without
+    Dirichlet faces, a wrong size and the adjoint are refused."""
+    _say("dirichlet errors")
+    pb = cube_problem(3, 2, bc=CUT)
+    u = smooth_state(pb, "euler", seed=1)
+    with Solver(pb, "euler") as s:
+        s.set_state(u)
+        with pytest.raises(RuntimeError, match="Dirichlet"):
+            s.step(1)
+        with pytest.raises(RuntimeError, match="Dirichlet"):
+            s.rhs(u)
+        with pytest.raises(ValueError):
+            s.set_dirichlet(np.zeros(7))
+        s.set_dirichlet(_face_states(pb, u, 0))
+        s.step(1)
+        with pytest.raises(RuntimeError, match="Dirichlet"):
+            s.adjoint_record(2)
+    with Solver(cube_problem(3, 2), "euler") as s:
+        assert len(s.dirichlet_faces()) == 0
+        with pytest.raises(RuntimeError, match="no Dirichlet"):
+            s.set_dirichlet(np.zeros((5, 0)))
