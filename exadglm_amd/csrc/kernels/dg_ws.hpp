@@ -10,12 +10,13 @@
 //            node values and geometry records straight into its LDS slot
 //            (global_load_lds_dwordx4, no registers), wait for those and for
 //            b+1's exterior traces (gathered a phase earlier), write tau;
-//   phase 2  consumers contract the volume rows of batch b and apply the
-//            LSRK update; producers form b+1's pressure and contravariant
-//            velocity V_d = sum_x g_dx v_x, its LM (or Rusanov) surface flux
-//            of all five variables at every face node (written once per face
-//            node into the surface buffer the consumers finished reading in
-//            phase 1), and issue the gathers of b+2's exterior traces.
+//   phase 2  consumers contract the volume rows of batch b, apply the LSRK
+//            update, then form b+1's pressure and contravariant velocity
+//            V_d = sum_x g_dx v_x (the producers are the longer path of this
+//            phase); producers form b+1's LM (or Rusanov) surface flux of all
+//            five variables at every face node (written once per face node
+//            into the surface buffer the consumers finished reading in
+//            phase 1) and issue the gathers of b+2's exterior traces.
 //
 // Consumer wave (v, t) owns row tile t (16 nodes) of variable v and keeps its
 // operator rows for the volume k-steps as v_mfma_f64_16x16x4f64 A-fragments
@@ -24,9 +25,9 @@
 // f_d = c V_d + p g_d,v-1  (c = rho, rho*u_v or E + p; the pressure term only
 // for the momenta), one node's value feeding its three direction k-steps
 // (contraction order k = d*Np4 + j).  The Np - 16*RTF rows that do not fill a
-// tile are accumulated on the VALU by the t = 0 waves (operator rows in LDS,
-// partial sums reduced across the four k-lanes with two shuffles) instead of
-// padding a whole MFMA tile.  Batches are visited in an order table (Morton
+// tile are accumulated on the VALU, split between the t = 0 and t = 1 waves of
+// the variable (operator rows in LDS, partial sums reduced across the four
+// k-lanes with two shuffles), instead of padding a whole MFMA tile.  Batches are visited in an order table (Morton
 // order of their centroids, built on the host) and the CUs of one XCD walk a
 // contiguous stretch of it, so the traces shared by neighbouring batches are
 // served from that XCD's L2.
@@ -143,12 +144,12 @@ __device__ inline void dma_copy(const double *src, double *dst, int bytes, int l
 
 // Role of hardware wave hw of the workgroup: r < NC is consumer (v, t) =
 // (r / RTF, r % RTF), r >= NC producer r - NC.  Waves are dealt to the four
-// SIMDs round-robin (SIMD hw % 4).  At N = 4 (16 waves: five consumers with
-// VALU remainder rows, five without, six producers) the table puts three
-// consumers and one producer on SIMDs 0 and 1 and two of each on SIMDs 2 and
-// 3, with the remainder-row consumers spread 1, 1, 2, 1, which evens out the
-// FP64 work (MFMA and VALU share it) per SIMD; otherwise roles follow the
-// wave index.
+// SIMDs round-robin (SIMD hw % 4).  At N = 4 (16 waves: ten consumers, the
+// t = 0 ones with two VALU remainder rows and the t = 1 ones with one, and six
+// producers) the table puts three consumers and one producer on SIMDs 0 and 1
+// and two of each on SIMDs 2 and 3, with the t = 0 consumers spread 1, 1, 2, 1,
+// which evens out the FP64 work (MFMA and VALU share it) per SIMD; otherwise
+// roles follow the wave index.
 template <int N>
 __device__ inline int ws_role(int hw) {
   using S = WsShape<N>;
@@ -377,11 +378,28 @@ __device__ inline void ws_surface(const StageArgs &a, const WsSlot<N> &sl, doubl
   }
 }
 
+// Remainder rows [lo, hi) of consumer tile t: the REM rows beyond the full
+// tiles are split between the first two tiles of a variable (both read the
+// same B-fragments), so that no single wave carries all of that VALU work
+// into the surface phase.
+template <int N>
+__host__ __device__ constexpr int ws_rem_lo(int t) {
+  using S = WsShape<N>;
+  constexpr int h = S::RTF >= 2 ? (S::REM + 1) / 2 : S::REM;
+  return t == 0 ? 0 : (t == 1 ? h : S::REM);
+}
+template <int N>
+__host__ __device__ constexpr int ws_rem_hi(int t) {
+  using S = WsShape<N>;
+  return t == 0 ? ws_rem_lo<N>(1) : S::REM;
+}
+
 // Consumer wave (v, t): batches b, b+gsz, ... < hi from alternating slots, two
-// barriers per batch.  WREM: this wave also accumulates the REM rows beyond
-// the full tiles (a separate instantiation so both loops stay straight-line
-// code).
-template <int N, int MODE, bool WREM>
+// barriers per batch.  It also accumulates the remainder rows [RLO, RHI) on
+// the VALU (a separate instantiation per range, so every loop stays
+// straight-line code), and forms the primitives of the next batch at the end
+// of the volume phase.
+template <int N, int MODE, int RLO, int RHI>
 __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const double *fs, const double *s_arem,
                                   const double *s_afs, const double *__restrict__ afrag, const int *order, double dt,
                                   int b, int hi, int gsz, int v, int t, int lane) {
@@ -389,7 +407,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
   using P = PrecTraits<PREC_F64>;
   constexpr int Np = S::Np, EB = S::EB, G = S::G, Np4 = S::Np4, KV = S::KV, NFF4 = S::NFF4;
   constexpr int KS = S::KS, KSS = S::KSS, KDP = S::KDP, RTF = S::RTF;
-  constexpr int REM = WREM ? S::REM : 0;
+  constexpr int REM = RHI - RLO;      // remainder rows of this wave
   constexpr int RA = REM > 0 ? REM : 1;
   constexpr int RR = (REM + 3) / 4;  // remainder rows per lane
   const int K = a.K;
@@ -423,7 +441,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
 #pragma unroll
       for (int x = 0; x < RR; x++) {
         const int rr = kk + 4 * x;
-        kpr[x] = (live && rr < REM) ? a.k[gbase + 16 * RTF + rr] : 0.0;
+        kpr[x] = (live && rr < REM) ? a.k[gbase + 16 * RTF + RLO + rr] : 0.0;
       }
     }
     // The remainder-row operator values are the same every batch; an opaque
@@ -431,7 +449,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
     // REM*KS registers held across it.
     int zoff = 0;
     asm volatile("" : "+v"(zoff));
-    const double *arw = s_arem + zoff + kk;
+    const double *arw = s_arem + zoff + RLO * KDP + kk;
 
     typename P::Acc acc = {0, 0, 0, 0};
     double racc[RA];
@@ -529,7 +547,7 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
         for (int r = 0; r < RA; r++)
           if (r == rr) rhs = racc[r];
         if (live && rr < REM) {
-          const int i = 16 * RTF + rr;
+          const int i = 16 * RTF + RLO + rr;
           const size_t gi = gbase + i;
           if (MODE == 1) {
             a.rhs[gi] = rhs;
@@ -541,6 +559,9 @@ __device__ inline void ws_consume(const StageArgs &a, WsSlot<N> *sl, const doubl
         }
       }
     }
+    // primitives of the next batch (its node values landed in the surface
+    // phase; the producers' surface fluxes do not read them)
+    if (b + gsz < hi) ws_primitives<N>(a, sl[(it + 1) & 1], (v * RTF + t) * 64 + lane, S::NC * 64);
     __syncthreads();  // slot free for batch b + 2*gsz
   }
 }
@@ -627,10 +648,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
         ws_land<N, DIR>(a, sl[(it + 1) & 1], s_cb[it & 1], pw, lane, pr);
       }
       __syncthreads();  // consumers are done with the surface buffer; batch nb staged
-      if (nb < hi) {
-        ws_primitives<N>(a, sl[(it + 1) & 1], pw * 64 + lane, S::NPW * 64);
-        ws_surface<N, DIR>(a, sl[(it + 1) & 1], s_fs, s_fmx, pb(nb), pw, lane, pr);
-      }
+      if (nb < hi) ws_surface<N, DIR>(a, sl[(it + 1) & 1], s_fs, s_fmx, pb(nb), pw, lane, pr);
       if (nnb < hi) ws_gather<N, DIR>(a, s_fmx, s_cb[it & 1], pb(nnb), pb(nnb + gsz), pw, lane, pr);
       __syncthreads();
     }
@@ -639,10 +657,17 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
   __syncthreads();  // batch b staged
   __syncthreads();  // batch b's primitives and surface fluxes complete
   const int v = w / RTF, t = w - v * RTF;
-  if (REM > 0 && t == 0)
-    ws_consume<N, MODE, true>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
-  else
-    ws_consume<N, MODE, false>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
+  constexpr int L0 = ws_rem_lo<N>(0), H0 = ws_rem_hi<N>(0), L1 = ws_rem_lo<N>(1), H1 = ws_rem_hi<N>(1);
+  if (t == 0) {
+    ws_consume<N, MODE, L0, H0>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
+  } else if constexpr (RTF <= 2) {
+    ws_consume<N, MODE, L1, H1>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
+  } else {
+    if (t == 1)
+      ws_consume<N, MODE, L1, H1>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
+    else
+      ws_consume<N, MODE, REM, REM>(a, sl, s_fs, s_arem, s_afs, afrag, order, dt, b, hi, gsz, v, t, lane);
+  }
 }
 
 // --------------------------------------------------------------------------
