@@ -58,6 +58,7 @@ class DGLM3DBase:
         self.filenames = []
         self._u0 = None
         self._velocity = None
+        self._dirichlet = None
         self.devid = 0
 
     def init_mesh(self, mesh_info=None, verbose=1):
@@ -120,6 +121,14 @@ class DGLM3DBase:
         self._u0 = np.asarray(u0, dtype=np.float64).reshape(self.Nvar, -1)
         self._velocity = velocity
 
+    def set_dirichlet(self, func):
+        """Boundary states of the Dirichlet faces (EtoB == 1; the reference's
+        calc_fluxLM leaves them as "ub = function of x,y,z,t",
+        euler_dglm.cpp:136-141): func(x, y, z, t) -> [Nvar, n] at the n face
+        nodes given.  run() then steps one step at a time and sets the states
+        at the time each step starts."""
+        self._dirichlet = func
+
     def _physics(self, filename):
         base = os.path.basename(filename)
         if base.startswith("advec"):
@@ -155,12 +164,30 @@ class DGLM3DBase:
             s.set_velocity(*self._velocity)
         s.set_state(u0)
         self._write_u(u0, 0, physics)
+        faces = s.dirichlet_faces()
+        if len(faces) and self._dirichlet is None:
+            s.close()
+            raise ValueError("the mesh has Dirichlet faces (EtoB == 1): give their states with set_dirichlet(func)")
+        if len(faces):
+            fmask = np.fromfile(os.path.join(self.datain_dir, "Fmask.bin"), dtype=np.int32).reshape(4, -1)
+            nodes = ((faces // 4)[:, None] * self.Np + fmask[faces % 4]).ravel()
+            fx, fy, fz = px[nodes], py[nodes], pz[nodes]
+
+        def advance(n):
+            if not len(faces):
+                return s.step(n, return_dt=True)
+            out = np.empty(n)
+            for i in range(n):
+                s.set_dirichlet(np.asarray(self._dirichlet(fx, fy, fz, s.time()[0]), dtype=np.float64))
+                out[i] = s.step(1, return_dt=True)[0]
+            return out
+
         # The reference loop (euler_main.cpp:8-15): step; print and write when
         # tstep is a multiple of the cadence or the step was clipped to TMAX.
         # Here the steps run in chunks that end on those multiples, with dt kept
         # on the device; the chunk's dt history gives the printed dt.
         lines = []
-        dts = s.step(1, return_dt=True)
+        dts = advance(1)
         print_tstep = max(1, find_print_tstep(self.tmax, dts[0], 30))
         while True:
             t, _, ts, done = s.time()
@@ -173,7 +200,7 @@ class DGLM3DBase:
                 self._write_u(s.get_state(), ts, physics)
             if done:
                 break
-            dts = s.step(print_tstep - ts % print_tstep, return_dt=True)
+            dts = advance(print_tstep - ts % print_tstep)
         s.close()
         elapsed = time.perf_counter() - start
         self.stdout = "\n".join(lines) + "\n"
