@@ -166,6 +166,38 @@ __device__ inline int ws_role(int hw) {
 
 __device__ inline int ws_batch(const int *order, int b) { return order ? order[b] : b; }
 
+// First face-node pair of producer pw (pw = NPW: the end); producer pw takes
+// pairs [ws_pair0(pw), ws_pair0(pw + 1)) of a batch.  A round of 64 pairs
+// costs the same VALU issue whether its lanes are full or not, so at N = 4
+// (960 pairs = 15 rounds) producers get whole rounds, ROUNDS_N4 per producer
+// (the role table puts producers 0..5 on SIMDs 0, 1, 2, 2, 3, 3); otherwise
+// an even split.
+#ifndef WS_ROUNDS_N4
+#define WS_ROUNDS_N4 2, 2, 3, 2, 3, 3
+#endif
+template <int N>
+__host__ __device__ constexpr int ws_pair0(int pw) {
+  using S = WsShape<N>;
+  if constexpr (S::NPAIR == 15 * 64 && S::NPW == 6) {
+    constexpr int r[6] = {WS_ROUNDS_N4};
+    int c = 0;
+    for (int i = 0; i < 6; i++) c += i < pw ? r[i] : 0;
+    return 64 * c;
+  } else {
+    return pw * S::PPW < S::NPAIR ? pw * S::PPW : S::NPAIR;
+  }
+}
+
+// Every producer's pairs fit its NRS register rounds, and together they cover
+// the batch.
+template <int N>
+constexpr bool ws_pairs_fit() {
+  using S = WsShape<N>;
+  for (int pw = 0; pw < S::NPW; pw++)
+    if (ws_pair0<N>(pw + 1) - ws_pair0<N>(pw) > 64 * ((S::PPW + 63) / 64)) return false;
+  return ws_pair0<N>(0) == 0 && ws_pair0<N>(S::NPW) == S::NPAIR;
+}
+
 // Dirichlet flags of a producer lane's pairs: bit s set when pair s lies on
 // a Dirichlet face (up[s] then holds the prescribed state).  Empty without
 // Dirichlet faces, so that instantiation's register state is unchanged.
@@ -204,13 +236,14 @@ template <int N, bool DIR>
 __device__ inline void ws_gather(const StageArgs &a, const int *s_fmx, const int *cb_cur, int b, int nb, int pw,
                                  int lane_in, WsProd<N, DIR> &pr) {
   using S = WsShape<N>;
-  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4, PPW = S::PPW;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4;
   constexpr int NRS = WsProd<N, DIR>::NRS;
   // lane-derived indices are the same every call: an opaque copy of the lane
   // id keeps them recomputed here rather than held in registers across the
   // batch loop
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
+  const int p0 = ws_pair0<N>(pw), np = ws_pair0<N>(pw + 1) - p0;  // this producer's pairs
   const int K = a.K;
   const size_t KNp = (size_t)K * Np;
   const int e0 = b * EB;
@@ -233,13 +266,13 @@ __device__ inline void ws_gather(const StageArgs &a, const int *s_fmx, const int
   if constexpr (DIR) pr.dm = 0;
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
-    const int pl = lane + 64 * s, P = PPW * pw + pl;
+    const int pl = lane + 64 * s, P = p0 + pl;
     const int q = P / NFF4, m = P - q * NFF4;
     // dead pairs load node 0 (never used): no select on the loaded values,
     // so nothing here waits for the loads
     int pidx = 0;  // K*Np < 2^31 (checked on the host)
     int ds = -1, jd = 0;
-    if (pl < PPW && P < S::NPAIR && q < ne && m < NFF) {
+    if (pl < np && q < ne && m < NFF) {
       const int f = m / Nfp, jf = m - f * Nfp;
       const int cc = cb_cur[q * 4 + f];
       pidx = conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
@@ -325,15 +358,16 @@ template <int N, bool DIR>
 __device__ inline void ws_surface(const StageArgs &a, const WsSlot<N> &sl, double *fs, const int *s_fmx, int b,
                                   int pw, int lane_in, const WsProd<N, DIR> &pr) {
   using S = WsShape<N>;
-  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4, PPW = S::PPW;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, EB = S::EB, NFF4 = S::NFF4;
   constexpr int NRS = WsProd<N, DIR>::NRS;
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
+  const int p0 = ws_pair0<N>(pw), np = ws_pair0<N>(pw + 1) - p0;  // this producer's pairs
   const int ne = min(EB, a.K - b * EB);
 #pragma unroll
   for (int s = 0; s < NRS; s++) {
-    const int pl = lane + 64 * s, P = PPW * pw + pl;
-    if (pl < PPW && P < S::NPAIR) {
+    const int pl = lane + 64 * s, P = p0 + pl;
+    if (pl < np) {
       const int e = P / NFF4, m = P - e * NFF4;
       double fl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (e < ne && m < NFF) {
@@ -580,6 +614,7 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
                                                                    int ngrp) {
   using S = WsShape<N>;
   constexpr int Np = S::Np, Nfp = S::Nfp, KDP = S::KDP, RTF = S::RTF, REM = S::REM, NC = S::NC, NT = S::NT;
+  static_assert(ws_pairs_fit<N>(), "producer pair ranges exceed their register rounds");
 
   __shared__ WsSlot<N> sl[2];
   __shared__ double s_fs[5 * S::NFF4 * S::FS];  // surface fluxes of the batch being consumed
