@@ -262,3 +262,67 @@ def test_c4_advection_adjoint_full_size(gpu):
     e = rel_err(g, diff)
     _say(f"  J={J0:.6e}  |M^-1 grad - (u0 - u1)| / |u0 - u1| = {e:.3e}")
     assert e < 1e-5  # measured 2.1e-6: the discrete adjoint is backward transport to that level
+
+
+def test_c5_euler_adjoint_full_size(gpu):
+    """The Euler adjoint configuration at full size: the C2 vortex (K = 196,608,
+    N = 4) with a 1e-3 Gaussian-bump field eps added to rho and p, J = int
+    1/2 rho |v|^2 at T after 300 steps (LM flux), sqrt(300) ~ 17 checkpoints,
+    gradient with respect to eps.  Through torch.autograd (eps -> u0 ->
+    dglm_evolve -> J): FP64 forward against central differences of the FP64
+    forward along 3 random bump fields, and the FP32-MFMA forward with the FP64
+    adjoint against the all-FP64 gradient."""
+    import torch
+
+    from exadglm_amd.autograd import dglm_evolve
+    from exadglm_amd.problem import gaussian_bumps, vortex_box
+    from exadglm_amd.workloads import GAMMA, c5, kinetic_energy, mass_weights, perturbed_state
+
+    w = c5()
+    pb = w["problem"]
+    a = pb.arrays
+    assert pb.K == 196608
+    steps, every = w["steps"], w["every"]
+    base = w["base"]
+    dev = "cuda"
+    rho, mx, my, mz, E = (torch.tensor(x, device=dev) for x in base)
+    q2 = (mx * mx + my * my + mz * mz) / (rho * rho)
+    p = (GAMMA - 1) * (E - 0.5 * rho * q2)
+    wq = torch.tensor(mass_weights(pb), device=dev)
+
+    def objective(solver, eps):
+        r2, p2 = rho + eps, p + eps
+        u0 = torch.stack([r2, r2 * mx / rho, r2 * my / rho, r2 * mz / rho, p2 / (GAMMA - 1) + 0.5 * r2 * q2])
+        uT = dglm_evolve(u0, solver, steps, every)
+        return (wq * 0.5 * (uT[1] ** 2 + uT[2] ** 2 + uT[3] ** 2) / uT[0]).sum()
+
+    grads = {}
+    for prec in ("fp64", "fp32"):
+        _say(f"C5 Euler adjoint K={pb.K} N=4 {steps} steps, {prec} forward")
+        with Solver(pb, "euler", cfl=w["cfl"], precision=prec) as s:
+            eps = torch.tensor(w["bump"], device=dev, requires_grad=True)
+            J = objective(s, eps)
+            J.backward()
+            grads[prec] = eps.grad.cpu().numpy()
+            if prec == "fp64":
+                # the C-ABI gradient and the numpy objective agree with the torch chain
+                u0, dud = perturbed_state(base, w["bump"])
+                s.set_state(u0)
+                s.set_time(0.0, 0)
+                s.adjoint_record(steps, every)
+                Jn, g = kinetic_energy(pb, s.get_state())
+                gu = s.adjoint_backward(g)
+                assert abs(Jn - float(J)) <= 1e-12 * abs(Jn)
+                assert rel_err((gu * dud).sum(axis=0), grads[prec]) < 1e-12
+                with torch.no_grad():
+                    for q in range(3):
+                        d = gaussian_bumps(a["px"], a["py"], a["pz"], 10 + q, vortex_box(32))
+                        h = 1e-3  # the bump amplitude is 1e-3: eps moves by ~1e-6
+                        dt = torch.tensor(d, device=dev)
+                        fd = (float(objective(s, eps + h * dt)) - float(objective(s, eps - h * dt))) / (2 * h)
+                        ad = float(np.dot(grads[prec], d))
+                        _say(f"  direction {q}: adjoint={ad:.12e} fd={fd:.12e}")
+                        assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+    e = rel_err(grads["fp32"], grads["fp64"])
+    _say(f"  FP32-forward gradient vs FP64: {e:.3e}")
+    assert e < 1e-5
