@@ -14,12 +14,13 @@
 //            volume node;
 //   partner  the same flux seen from the neighbour element depends on this
 //            element's node through its u+:  d/du+ = Fscale_nbr (J_nbr(ub)/2
-//            - tau/2).  Each element PULLS these terms from its neighbours'
-//            stored face adjoints through the partner slot (exact by the
-//            involution of the face connectivity) -- no atomics, so the
-//            adjoint is bitwise deterministic.
+//            - tau/2).  The neighbour's pass A forms it beside its own-side
+//            term (the same ub, normal, h and tau) and stores it at its face
+//            slot; each element PULLS these terms through the partner slot
+//            (exact by the involution of the face connectivity) -- no
+//            atomics, so the adjoint is bitwise deterministic.
 // Pass A computes A^T rbar, the volume and own-face terms and tau-bar, and
-// stores the face part of A^T rbar; pass B adds the partner terms.
+// stores the partner-side face terms; pass B adds them.
 #pragma once
 #include "dg_fp64.hpp"
 
@@ -59,7 +60,7 @@ struct AdjArgs {
   double *ubar;        // [NV][K*Np]  in/out
   double *kbar;        // [NV][K*Np]  in/out (mode 0)
   const double *rbar;  // [NV][K*Np]  (mode 1)
-  double *gsur;        // [K][NFF][NV] face part of A^T rbar (pass A out, pass B in)
+  double *gsur;        // [K][NFF][NV] partner-side face terms (pass A out, pass B in)
   double *taubar;      // [K*4] accumulated tau adjoint per face side (may be null)
   const double *vel;
   const double *geo;
@@ -218,11 +219,18 @@ __global__ __launch_bounds__(256) void adjoint_pass_a(AdjArgs a) {
           h[v] = 0.5 * g[3] * s_g[v][e * KD + 3 * Np + mslot];
         }
         euler_jt(ub, g[0], g[1], g[2], h, a.gamma, w);
-        for (int v = 0; v < 5; v++) s_w[v][idx] += w[v] + tau * h[v];
+        // own side: (1/2 n.J + tau/2)^T Fscale g; the partner's node gets
+        // (1/2 n.J - tau/2)^T Fscale g, the same terms, stored for pass B
+        double *pt = &a.gsur[((size_t)(e0 + e) * NFF + mslot) * NV];
+        for (int v = 0; v < 5; v++) {
+          s_w[v][idx] += w[v] + tau * h[v];
+          pt[v] = w[v] - tau * h[v];
+        }
       } else {
         const double gg = s_g[0][e * KD + 3 * Np + mslot];
         const double vn = g[0] * s_vel[0][idx] + g[1] * s_vel[1][idx] + g[2] * s_vel[2][idx];
         s_w[0][idx] += g[3] * (0.5 * vn + 0.5 * tau) * gg;
+        a.gsur[(size_t)(e0 + e) * NFF + mslot] = g[3] * (0.5 * vn - 0.5 * tau) * gg;
       }
     }
   }
@@ -249,21 +257,17 @@ __global__ __launch_bounds__(256) void adjoint_pass_a(AdjArgs a) {
 
   for (int v = 0; v < NV; v++)
     for (int i = tid; i < ne * Np; i += 256) a.ubar[v * KNp + (size_t)e0 * Np + i] = s_w[v][i];
-  for (int idx = tid; idx < ne * NFF; idx += 256) {
-    const int e = idx / NFF, m = idx - e * NFF;
-    for (int v = 0; v < NV; v++) a.gsur[((size_t)(e0 + e) * NFF + m) * NV + v] = s_g[v][e * KD + 3 * Np + m];
-  }
 }
 
 // Pass B: one thread per volume node; pull the partner-side terms of the
-// (at most 3) faces the node lies on.
+// (at most 3) faces the node lies on.  The partner element's pass A formed
+// them (same ub, normal, h and tau as its own-side term) and stored them at
+// its face-node slot, so only the connectivity, that slot and ubar are read.
 template <int N, int PHYS>
 __global__ __launch_bounds__(256) void adjoint_pass_b(AdjArgs a) {
   using O = Ord<N>;
   constexpr int Np = O::Np, Nfp = O::Nfp, NFF = O::NFF;
   constexpr int NV = NVar<PHYS>::value;
-  // the per-order tables in LDS: only the connectivity and the partner
-  // values remain global loads, two round trips per node
   __shared__ int s_slots[Np * 3], s_fmx[16 * Nfp], s_fpos[4 * Np];
   for (int i = threadIdx.x; i < Np * 3; i += 256) s_slots[i] = a.node_slots[i];
   for (int i = threadIdx.x; i < 16 * Nfp; i += 256) s_fmx[i] = a.fmx[i];
@@ -274,56 +278,30 @@ __global__ __launch_bounds__(256) void adjoint_pass_b(AdjArgs a) {
   const int idx = blockIdx.x * 256 + threadIdx.x;
   if (idx >= K * Np) return;
   const int e = idx / Np, i = idx - e * Np;
-  double w[NV];
-  for (int v = 0; v < NV; v++) w[v] = 0.0;
-  // all loads of the (at most 3) partner slots first, then the arithmetic
-  int ms[3];
-  size_t pidx[3];
-  const double *gn[3], *gs[3];
-  double tau_n[3];
+  // all loads of the (at most 3) partner slots first, then the sums
+  double t[3][NV];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    ms[q] = s_slots[i * 3 + q];
-    if (ms[q] >= 0) {
-      const int f = ms[q] / Nfp, j = ms[q] - f * Nfp;
+    const int ms = s_slots[i * 3 + q];
+#pragma unroll
+    for (int v = 0; v < NV; v++) t[q][v] = 0.0;
+    if (ms >= 0) {
+      const int f = ms / Nfp, j = ms - f * Nfp;
       const int c = a.conn[(size_t)e * 4 + f];
       const int nb = conn_nbr(c), nf = conn_face(c);
       const int pnode = s_fmx[(nf * 4 + conn_code(c)) * Nfp + j];  // neighbour's volume node
       const int jp = s_fpos[nf * Np + pnode];                       // its slot on face nf
-      pidx[q] = (size_t)nb * Np + pnode;
-      gn[q] = &a.geo[(size_t)nb * GEO_STRIDE + 9 + nf * 4];
-      tau_n[q] = 0.25 * (a.maxv[(size_t)nb * 4 + nf] + a.maxv[(size_t)e * 4 + f]) * a.tau_scale;
-      gs[q] = &a.gsur[((size_t)nb * NFF + nf * Nfp + jp) * NV];
+      const double *gs = &a.gsur[((size_t)nb * NFF + nf * Nfp + jp) * NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) t[q][v] = gs[v];
     }
   }
-  if (PHYS == PHYS_EULER) {
-    double uo[5];
+  for (int v = 0; v < NV; v++) {
+    double w = 0.0;
 #pragma unroll
-    for (int v = 0; v < 5; v++) uo[v] = a.u[v * KNp + idx];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      if (ms[q] < 0) break;
-      double ub[5], h[5];
-#pragma unroll
-      for (int v = 0; v < 5; v++) {
-        ub[v] = 0.5 * (a.u[v * KNp + pidx[q]] + uo[v]);
-        h[v] = 0.5 * gn[q][3] * gs[q][v];
-      }
-      double ww[5] = {0, 0, 0, 0, 0};
-      euler_jt(ub, gn[q][0], gn[q][1], gn[q][2], h, a.gamma, ww);
-#pragma unroll
-      for (int v = 0; v < 5; v++) w[v] += ww[v] - tau_n[q] * h[v];
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-      if (ms[q] < 0) break;
-      const size_t pi = pidx[q];
-      const double vn = gn[q][0] * a.vel[pi] + gn[q][1] * a.vel[KNp + pi] + gn[q][2] * a.vel[2 * KNp + pi];
-      w[0] += gn[q][3] * (0.5 * vn - 0.5 * tau_n[q]) * gs[q][0];
-    }
+    for (int q = 0; q < 3; q++) w += t[q][v];
+    a.ubar[v * KNp + idx] += w;
   }
-  for (int v = 0; v < NV; v++) a.ubar[v * KNp + idx] += w[v];
 }
 
 // dtbar of the step = sum of the block partials + tbar (adjoint of the time

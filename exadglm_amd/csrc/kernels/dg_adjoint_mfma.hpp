@@ -16,8 +16,8 @@
 //   * phase 2 is pointwise per volume node: the transposed volume Jacobians,
 //     the node's own-face terms (exterior traces gathered from global memory
 //     ahead of the volume terms) and the write of ubar; each face-node slot
-//     leaves its tau-bar partial in LDS, summed per face side afterwards; then
-//     the face part of g for pass B.
+//     stores the partner-side term its neighbour's node needs (pass B) and
+//     leaves its tau-bar partial in LDS, summed per face side afterwards.
 #pragma once
 #include "dg_adjoint.hpp"
 #include "dg_mfma.hpp"
@@ -225,9 +225,12 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
       s_tc[e * NFF + mslot] = tc;
       euler_jt(ub, g[0], g[1], g[2], h, a.gamma, ww);
       double fw = 0.0;
+      // the partner's node gets (1/2 n.J - tau/2)^T Fscale g: stored for pass B
+      double *pt = &a.gsur[((size_t)(e0 + e) * NFF + mslot) * 5];
 #pragma unroll
       for (int v = 0; v < 5; v++) {
         wv[v] += ww[v] + tau * h[v];
+        pt[v] = ww[v] - tau * h[v];
         fw += ww[v] * ub[v];
       }
       rdot += 2.0 * fw + tau * tc;
@@ -260,11 +263,6 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
       const size_t o = (size_t)e0 * 4 + idx;
       a.taubar[o] = (a.mode == 0 ? a.taubar[o] : 0.0) + t;
     }
-  // face part of g for pass B, [K][NFF][5]
-  for (int idx = tid; idx < ne * NFF * 5; idx += NT) {
-    const int el = idx / (NFF * 5), r = idx - el * NFF * 5, mm = r / 5, v = r - mm * 5;
-    a.gsur[(size_t)(e0 + el) * NFF * 5 + r] = s_g[v][(3 * Np + mm) * EB + el];
-  }
 }
 
 }  // namespace dglm
