@@ -198,6 +198,49 @@ def adjoint_leg(precision="fp64", nhex=32, order=4, steps=300):
             "tape_bytes": tape, "K": pb.K, "every": every}
 
 
+def hw_check():
+    """Device figures behind the roofline peaks, measured: compute units and
+    LDS per workgroup from the runtime, HBM bandwidth from a 4 GiB device copy
+    (read + write) and a 4 GiB read-only sum, and the rate of large library
+    GEMMs (8192^3; a lower bound on the dense MFMA peak)."""
+    import torch
+
+    dev = torch.cuda.current_device()
+    prop = torch.cuda.get_device_properties(dev)
+    out = {"name": prop.name, "compute_units": prop.multi_processor_count,
+           "lds_per_block_bytes": getattr(prop, "shared_memory_per_block", None),
+           "hbm_total_bytes": prop.total_memory}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    n = (4 << 30) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    t = timed(lambda: dst.copy_(src), 10)
+    out["hbm_copy_GBs"] = 2 * n * 8 / t / 1e9
+    t = timed(lambda: src.sum(), 10)
+    out["hbm_read_GBs"] = n * 8 / t / 1e9
+    del src, dst
+    m = 8192
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        a = torch.rand(m, m, device="cuda").to(dt)
+        b = torch.rand(m, m, device="cuda").to(dt)
+        t = timed(lambda: torch.mm(a, b), 5)
+        out[f"gemm_{name}_TFs"] = 2 * m ** 3 / t / 1e12
+        del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +261,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a profiled run to bytes)")
+    ap.add_argument("--hw-check", action="store_true",
+                    help="only measure the device figures the rooflines assume (CUs, LDS, HBM copy and read "
+                         "bandwidth, library GEMM rates in FP64/FP32/BF16) and print them")
     ap.add_argument("--adjoint-only", type=int, default=0, metavar="STEPS",
                     help="only run the FP64 Euler adjoint leg with STEPS steps (for a profiler) and print its times")
     args = ap.parse_args()
@@ -227,6 +273,9 @@ def main():
         res["ratio_record_plus_backward_over_forward"] = (res["t_record_s"] + res["t_backward_s"]) / \
             res["t_forward_s"]
         print(json.dumps(res), flush=True)
+        return
+    if args.hw_check:
+        print(json.dumps(hw_check()), flush=True)
         return
     if args.calibrate_copy:
         import torch
