@@ -264,6 +264,9 @@ def main():
     ap.add_argument("--hw-check", action="store_true",
                     help="only measure the device figures the rooflines assume (CUs, LDS, HBM copy and read "
                          "bandwidth, library GEMM rates in FP64/FP32/BF16) and print them")
+    ap.add_argument("--mixed-only", type=int, default=0, metavar="STEPS",
+                    help="only run the FP64 and mixed-precision N=6 vortex legs with STEPS timed steps each "
+                         "(for a profiler) and print their times")
     ap.add_argument("--adjoint-only", type=int, default=0, metavar="STEPS",
                     help="only run the FP64 Euler adjoint leg with STEPS steps (for a profiler) and print its times")
     args = ap.parse_args()
@@ -273,6 +276,10 @@ def main():
         res["ratio_record_plus_backward_over_forward"] = (res["t_record_s"] + res["t_backward_s"]) / \
             res["t_forward_s"]
         print(json.dumps(res), flush=True)
+        return
+    if args.mixed_only:
+        res = mixed_precision_leg(steps=args.mixed_only, warmup=2)
+        print(json.dumps({k: res[k] for k in ("workload", "fp64", "mixed", "speedup")}), flush=True)
         return
     if args.hw_check:
         print(json.dumps(hw_check()), flush=True)
