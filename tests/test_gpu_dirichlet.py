@@ -235,3 +235,25 @@ def test_dirichlet_mixed_precision(gpu, N):
     met = max(np.abs(a[k]).max() for k in ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "Fscale"))
     for v in range(5):
         assert np.abs(res["fp32"][v] - ro[v]).max() < 1e-7 * 3 * opn * met * np.abs(u).max() ** 2
+
+
+def test_dirichlet_device_states(gpu):
+    """dglm_set_dirichlet_device: states in a device buffer ([nvar][nd][Nfp],
+    a torch CUDA tensor here) give the same RHS, bitwise, as the host call."""
+    import torch
+
+    _say("dirichlet device states")
+    pb = cube_problem(3, 3, bc=CUT)
+    u = noisy_state(pb, "euler", seed=4, eps=0.02)
+    g = _face_states(pb, u, seed=3)
+    with Solver(pb, "euler") as s:
+        faces = s.dirichlet_faces()
+        gc = np.ascontiguousarray(g.reshape(5, pb.K * 4, pb.Nfp)[:, faces, :])
+        s.set_dirichlet(gc)
+        r_host = s.rhs(u)[0]
+        gd = torch.tensor(gc, dtype=torch.float64, device="cuda")
+        s.set_dirichlet(np.zeros_like(gc))  # something else first
+        torch.cuda.synchronize()
+        s.set_dirichlet_device(gd.data_ptr())
+        r_dev = s.rhs(u)[0]
+    np.testing.assert_array_equal(r_dev, r_host)
