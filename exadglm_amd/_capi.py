@@ -76,7 +76,8 @@ EXPORTS = [
     "dglm_get_state_device", "dglm_set_velocity", "dglm_set_time", "dglm_get_time", "dglm_rhs", "dglm_step",
     "dglm_integrals", "dglm_sync", "dglm_set_timing", "dglm_get_timing",
     "dglm_set_stage_kernel", "dglm_get_stage_kernel", "dglm_adjoint_rhs", "dglm_adjoint_record",
-    "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_tape_bytes", "dglm_set_graphs",
+    "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_set_keep", "dglm_adjoint_tape_bytes",
+    "dglm_set_graphs",
     "dglm_dirichlet_faces", "dglm_set_dirichlet", "dglm_set_dirichlet_device",
 ]
 

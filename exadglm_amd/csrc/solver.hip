@@ -88,6 +88,7 @@ struct dglm_ctx {
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // keyed by the state buffer the step starts from
   double *d_J = nullptr, *d_w = nullptr, *d_ipart = nullptr;  // conservation sums
   bool adj_exact = true;          // include the tau(u) and dt(u) sensitivities (Euler)
+  int adj_keep_max = -1;          // segments kept whole on the tape (-1: as many as half the free HBM holds)
   std::vector<int> h_conn;  // host copy (tau pairing for the rhs outputs)
   // Dirichlet faces (EtoB == 1): slot map [K*4] (-1: none), prescribed states
   // [nvar][nd][Nfp] and the faces of the slots in ascending order.
@@ -1249,7 +1250,12 @@ struct dglm_adj_tape {
   double *ubar = nullptr, *kbar = nullptr, *gsur = nullptr, *taubar = nullptr, *dtpart = nullptr, *tsc = nullptr;
   double *rbar = nullptr;
   int *slots = nullptr, *fpos = nullptr;
-  size_t bytes = 0;
+  // The last nkeep segments kept whole: their stage states [5*every + 1][nst]
+  // and wave speeds [every][K*4] per segment, written by the recording
+  // forward itself, so the reverse sweep does not recompute them.
+  double *keep = nullptr, *keep_lam = nullptr;
+  int nkeep = 0;
+  size_t bytes = 0, keep_bytes = 0;
 };
 
 namespace {
@@ -1274,7 +1280,7 @@ void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step) {
 
 void free_tape(dglm_adj_tape *T) {
   if (!T) return;
-  void *ptrs[] = {T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2,
+  void *ptrs[] = {T->keep, T->keep_lam, T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2,
                   T->ubar, T->kbar, T->gsur, T->taubar, T->dtpart, T->tsc, T->rbar, T->slots, T->fpos,
                   T->afragT};
   for (void *q : ptrs)
@@ -1435,17 +1441,67 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
     T->nsteps = nsteps;
     T->every = every;
     T->nck = nck;
+    if (T->keep) HIP_TRY(hipFree(T->keep));
+    if (T->keep_lam) HIP_TRY(hipFree(T->keep_lam));
+    T->keep = T->keep_lam = nullptr;
+    T->keep_bytes = 0;  // T->bytes is recomputed below without them
+    T->nkeep = -1;      // sized below
     T->bytes = (2 * nst + (size_t)c->nvar * c->K * c->NFF + (size_t)c->K * 4 + T->nblk_a) * sizeof(double) +
                (nst * (nck + 5 * (size_t)every + 2) + (size_t)c->K * 4 * every) * sizeof(double);
   }
+  // Segments kept whole: HBM bought back from the recompute.  The count is the
+  // caller's cap (dglm_adjoint_set_keep) or, by default, what half of the free
+  // device memory holds; sized once per tape shape.
+  const size_t seg_n = nst * (5 * (size_t)every + 1), lam_n = (size_t)c->K * 4 * every;
+  if (T->nkeep < 0 || (c->adj_keep_max >= 0 && T->nkeep > c->adj_keep_max)) {
+    if (T->keep) HIP_TRY(hipFree(T->keep));
+    if (T->keep_lam) HIP_TRY(hipFree(T->keep_lam));
+    T->keep = T->keep_lam = nullptr;
+    T->bytes -= T->keep_bytes;
+    T->keep_bytes = 0;
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    const size_t per = (seg_n + lam_n) * sizeof(double);
+    long nk = (long)((fr / 2) / per);
+    if (c->adj_keep_max >= 0 && nk > c->adj_keep_max) nk = c->adj_keep_max;
+    if (nk > nck) nk = nck;
+    T->nkeep = (int)nk;
+    if (T->nkeep > 0) {
+      if ((rc = dev_alloc(&T->keep, seg_n * T->nkeep, "kept segments")) ||
+          (rc = dev_alloc(&T->keep_lam, lam_n * T->nkeep, "kept wave speeds"))) {
+        T->nkeep = -1;
+        return rc;
+      }
+      T->keep_bytes = per * T->nkeep;
+      T->bytes += T->keep_bytes;
+    }
+  }
+  const int q0 = nck - T->nkeep;  // first kept segment
   const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
   for (int n = 0; n < nsteps; n++) {
+    const int q = n / every;
     if (n % every == 0) {
-      const int q = n / every;
       HIP_TRY(hipMemcpyAsync(T->ckpt + nst * q, c->u[c->cur], nst * sizeof(double), hipMemcpyDeviceToDevice,
                              c->stream));
       HIP_TRY(hipMemcpyAsync(T->ck_t + q, &c->scal->t_next, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       HIP_TRY(hipMemcpyAsync(T->ck_step + q, &c->scal->tstep, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (q >= q0) {
+      // a kept segment: the stages write straight into its tape slots (the
+      // same kernels and inputs as the reverse sweep's recompute)
+      const int n0 = q * every, n1 = std::min(n0 + every, nsteps);
+      double *base = T->keep + seg_n * (q - q0) + nst * 5 * (size_t)(n - n0);
+      double *lam = T->keep_lam + lam_n * (q - q0) + (size_t)c->K * 4 * (n - n0);
+      if (n == n0)
+        HIP_TRY(hipMemcpyAsync(base, c->u[c->cur], nst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      ws(c, base, lam);
+      hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
+                         c->nblk_ws, c->scal, cfl_fac, c->p.tmax, T->dt, T->fstar, T->clip, n);
+      for (int s = 0; s < 5; s++) st0(c, make_stage_args(c, base + nst * s, base + nst * (s + 1), c->k, lam, c->scal, s));
+      if (n == n1 - 1)  // the context's state continues from the segment's last stage
+        HIP_TRY(hipMemcpyAsync(c->u[c->cur], base + nst * 5, nst * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+      continue;
     }
     ws(c, c->u[c->cur], c->maxv);
     hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
@@ -1457,6 +1513,13 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
     }
   }
   HIP_TRY(hipGetLastError());
+  return DGLM_OK;
+}
+
+extern "C" int dglm_adjoint_set_keep(dglm_ctx *c, int max_segments) {
+  if (!c) return fail(DGLM_ERR_ARG, "dglm_adjoint_set_keep: null context");
+  c->adj_keep_max = max_segments < 0 ? -1 : max_segments;
+  if (c->tape) c->tape->nkeep = -1;  // resized at the next recording
   return DGLM_OK;
 }
 
@@ -1482,27 +1545,33 @@ extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *
   HIP_TRY(hipMemsetAsync(T->kbar, 0, nst * sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(T->kscr, 0, nst * sizeof(double), c->stream));
   HIP_TRY(hipMemsetAsync(T->tsc, 0, 4 * sizeof(double), c->stream));
+  const int q0 = T->nck - std::max(T->nkeep, 0);  // first segment kept whole by the recording
   for (int q = T->nck - 1; q >= 0; q--) {
     const int n0 = q * T->every, n1 = std::min(n0 + T->every, T->nsteps);
-    // recompute the segment's stage states from its checkpoint
-    DevScalars s0{};
-    HIP_TRY(hipMemcpyAsync(T->seg, T->ckpt + nst * q, nst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipMemsetAsync(T->scal2, 0, sizeof(DevScalars), c->stream));
-    HIP_TRY(hipMemcpyAsync(&T->scal2->t_next, T->ck_t + q, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    (void)s0;
-    for (int n = n0; n < n1; n++) {
-      double *base = T->seg + nst * 5 * (size_t)(n - n0);
-      double *lam = T->lam + (size_t)c->K * 4 * (n - n0);
-      ws(c, base, lam);
-      hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
-                         c->nblk_ws, T->scal2, cfl_fac, c->p.tmax, (double *)nullptr, (int *)nullptr, (int *)nullptr,
-                         0);
-      for (int s = 0; s < 5; s++) st0(c, make_stage_args(c, base + nst * s, base + nst * (s + 1), T->kscr, lam, T->scal2, s));
+    double *seg = T->seg, *segl = T->lam;
+    if (q >= q0) {
+      seg = T->keep + nst * (5 * (size_t)T->every + 1) * (q - q0);
+      segl = T->keep_lam + (size_t)c->K * 4 * T->every * (q - q0);
+    } else {
+      // recompute the segment's stage states from its checkpoint
+      HIP_TRY(hipMemcpyAsync(T->seg, T->ckpt + nst * q, nst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemsetAsync(T->scal2, 0, sizeof(DevScalars), c->stream));
+      HIP_TRY(hipMemcpyAsync(&T->scal2->t_next, T->ck_t + q, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      for (int n = n0; n < n1; n++) {
+        double *base = T->seg + nst * 5 * (size_t)(n - n0);
+        double *lam = T->lam + (size_t)c->K * 4 * (n - n0);
+        ws(c, base, lam);
+        hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
+                           c->nblk_ws, T->scal2, cfl_fac, c->p.tmax, (double *)nullptr, (int *)nullptr,
+                           (int *)nullptr, 0);
+        for (int s = 0; s < 5; s++)
+          st0(c, make_stage_args(c, base + nst * s, base + nst * (s + 1), T->kscr, lam, T->scal2, s));
+      }
     }
     // reverse sweep over the segment
     for (int n = n1 - 1; n >= n0; n--) {
-      double *base = T->seg + nst * 5 * (size_t)(n - n0);
-      const double *lam = T->lam + (size_t)c->K * 4 * (n - n0);
+      double *base = seg + nst * 5 * (size_t)(n - n0);
+      const double *lam = segl + (size_t)c->K * 4 * (n - n0);
       HIP_TRY(hipMemsetAsync(T->taubar, 0, (size_t)c->K * 4 * sizeof(double), c->stream));
       HIP_TRY(hipMemsetAsync(T->dtpart, 0, (size_t)T->nblk_a * sizeof(double), c->stream));
       for (int s = 4; s >= 0; s--) {

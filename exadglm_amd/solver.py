@@ -220,6 +220,11 @@ class Solver:
     def set_adjoint_exact(self, on=True):
         A.check(A.lib().dglm_adjoint_set_exact(self._ctx, 1 if on else 0))
 
+    def set_adjoint_keep(self, max_segments=-1):
+        """Checkpoint segments the recording keeps whole (no recompute in the
+        reverse sweep): -1 = as many as half the free device memory holds, 0 = none."""
+        A.check(A.lib().dglm_adjoint_set_keep(self._ctx, int(max_segments)))
+
     def tape_bytes(self):
         b = C.c_double()
         A.check(A.lib().dglm_adjoint_tape_bytes(self._ctx, C.byref(b)))

@@ -232,6 +232,15 @@ int dglm_adjoint_backward(dglm_ctx *ctx, const double *ubar_T, double *ubar_0, i
 /* on = 0 freezes tau and dt at their forward values (the exact gradient of
  * the frozen-coefficient map); default 1 (exact; only Euler depends on it). */
 int dglm_adjoint_set_exact(dglm_ctx *ctx, int on);
+/* Checkpoint segments kept whole on the tape: the recording forward writes
+ * the stage states and wave speeds of the last `max_segments` segments
+ * straight into device memory, and the reverse sweep skips their recompute
+ * (the gradient is bitwise the same).  -1 (default): as many as half of the
+ * free device memory holds at the recording; 0: none.  Takes effect at the
+ * next dglm_adjoint_record.  No reference counterpart (the reference has no
+ * adjoint). */
+int dglm_adjoint_set_keep(dglm_ctx *ctx, int max_segments);
+/* Device memory of the adjoint buffers and tape, kept segments included. */
 int dglm_adjoint_tape_bytes(dglm_ctx *ctx, double *bytes);
 
 #ifdef __cplusplus

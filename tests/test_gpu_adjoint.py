@@ -326,3 +326,26 @@ def test_c5_euler_adjoint_full_size(gpu):
     e = rel_err(grads["fp32"], grads["fp64"])
     _say(f"  FP32-forward gradient vs FP64: {e:.3e}")
     assert e < 1e-5
+
+
+def test_kept_segments_do_not_change_gradient(gpu):
+    """Segments kept whole by the recording (no recompute in the reverse sweep)
+    give the gradient bitwise: none kept, two kept, and the default (as many
+    as half the free memory holds, here all of them); the tape grows with them
+    and the state after the recording is the same."""
+    _say("kept segments")
+    pb, u0 = _euler_setup()
+    wts = np.random.default_rng(7).uniform(0.5, 1.5, u0.shape)
+    res = {}
+    with Solver(pb, "euler", cfl=0.4) as s:
+        for keep in (0, 2, -1):
+            s.set_adjoint_keep(keep)
+            s.set_state(u0)
+            s.set_time(0.0, 0)
+            s.adjoint_record(20, 4)
+            uT = s.get_state()
+            res[keep] = (uT, s.adjoint_backward(wts * uT), s.tape_bytes())
+    for keep in (2, -1):
+        np.testing.assert_array_equal(res[keep][0], res[0][0])
+        np.testing.assert_array_equal(res[keep][1], res[0][1])
+    assert res[0][2] < res[2][2] < res[-1][2]
