@@ -1467,13 +1467,19 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
     if (nk > nck) nk = nck;
     T->nkeep = (int)nk;
     if (T->nkeep > 0) {
-      if ((rc = dev_alloc(&T->keep, seg_n * T->nkeep, "kept segments")) ||
-          (rc = dev_alloc(&T->keep_lam, lam_n * T->nkeep, "kept wave speeds"))) {
-        T->nkeep = -1;
-        return rc;
+      if (hipMalloc((void **)&T->keep, seg_n * T->nkeep * sizeof(double) + 16) != hipSuccess ||
+          hipMalloc((void **)&T->keep_lam, lam_n * T->nkeep * sizeof(double) + 16) != hipSuccess) {
+        // the budget is an estimate: without the memory, keep nothing and
+        // recompute every segment (clear the allocation error first)
+        (void)hipGetLastError();
+        if (T->keep) hipFree(T->keep);
+        if (T->keep_lam) hipFree(T->keep_lam);
+        T->keep = T->keep_lam = nullptr;
+        T->nkeep = 0;
+      } else {
+        T->keep_bytes = per * T->nkeep;
+        T->bytes += T->keep_bytes;
       }
-      T->keep_bytes = per * T->nkeep;
-      T->bytes += T->keep_bytes;
     }
   }
   const int q0 = nck - T->nkeep;  // first kept segment
