@@ -78,7 +78,7 @@ EXPORTS = [
     "dglm_set_stage_kernel", "dglm_get_stage_kernel", "dglm_adjoint_rhs", "dglm_adjoint_record",
     "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_set_keep", "dglm_adjoint_tape_bytes",
     "dglm_set_graphs",
-    "dglm_dirichlet_faces", "dglm_set_dirichlet", "dglm_set_dirichlet_device",
+    "dglm_dirichlet_faces", "dglm_set_dirichlet", "dglm_set_dirichlet_device", "dglm_set_alloc_faults",
 ]
 
 _lib = None

@@ -5,10 +5,16 @@
     J = objective(uT); J.backward()                 # u0.grad = (dS/du0)^T dJ/duT
 
 The forward runs the HIP step loop and keeps checkpoints every `every` steps
-(default ~sqrt(nsteps)); backward recomputes each segment from its checkpoint
-and runs the FP64 adjoint kernels (including the tau(u) and dt(u)
+(default ~sqrt(nsteps)), the last segments whole where device memory allows
+(Solver.set_adjoint_keep); backward recomputes every other segment from its
+checkpoint and runs the FP64 adjoint kernels (including the tau(u) and dt(u)
 sensitivities for Euler).  A solver created with precision="fp32" gives an
 FP32-MFMA forward with the FP64 adjoint along the same trajectory.
+
+A solver holds one recorded trajectory: a second dglm_evolve on the same
+solver replaces the first one's recording, and backward through the first
+then raises instead of returning the gradient of the second trajectory (use
+one solver per differentiable evolve in a graph).
 """
 import torch
 
@@ -28,10 +34,14 @@ class DGLMEvolve(torch.autograd.Function):
         out = torch.empty_like(u0c)
         solver.get_state_device(out.data_ptr())  # synchronises the solver stream
         ctx.solver = solver
+        ctx.recording = solver.recording
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
+        if ctx.solver.recording != ctx.recording:
+            raise RuntimeError("dglm_evolve: the solver's tape was re-recorded by a later dglm_evolve before this "
+                               "backward; one solver holds one recorded trajectory (use a solver per evolve)")
         g = grad_out.contiguous()
         gin = torch.empty_like(g)
         torch.cuda.current_stream().synchronize()

@@ -1,9 +1,10 @@
-// FP64 DGLM kernels: wave speed, on-device dt, and the fused per-stage
+// FP64 DGLM kernels: pointwise Euler fluxes and the fused per-stage
 // RHS + LSRK kernel (volume flux, LM surface flux, stacked contraction with
 // [WDr|WDs|WDt|-LIFT], LSRK register update) -- one launch per stage.
 //
 // Math follows the reference CPU path:
 //   calc_maxv   dim3/src/cpp/euler_dglm.cpp:6-50,   advec_dglm.cpp:6-35
+//               (its kernel, wavespeed_lds_kernel, is in dg_ws.hpp)
 //   calc_dt     euler_dglm.cpp:53-69 (its kernel is in dg_step.hpp)
 //   update_tau  euler_dglm.cpp:72-84
 //   update_ub + calc_fluxLM  euler_dglm.cpp:87-183, advec_dglm.cpp:69-133
@@ -92,67 +93,6 @@ __device__ inline void euler_rusanov5(const double uM[5], const double uP[5], co
   const double lam = fmax(lm, lp);
 #pragma unroll
   for (int v = 0; v < 5; v++) fl[v] = 0.5 * (fm[v] + fp[v]) + 0.5 * lam * (uM[v] - uP[v]);
-}
-
-// --------------------------------------------------------------------------
-// Wave speed per face + block maximum of lambda*Fscale
-// --------------------------------------------------------------------------
-template <int N, int PHYS>
-__global__ __launch_bounds__(256) void wavespeed_kernel(int K, const double *__restrict__ u,
-                                                        const double *__restrict__ vel,
-                                                        const double *__restrict__ geo, const int *__restrict__ fmx,
-                                                        double *__restrict__ maxv, double *__restrict__ partial,
-                                                        int *__restrict__ partial_idx, double gamma) {
-  using O = Ord<N>;
-  constexpr int Np = O::Np, Nfp = O::Nfp;
-  const size_t KNp = (size_t)K * Np;
-  __shared__ double red[256];
-  __shared__ int red_i[256];
-  const int fid = blockIdx.x * 256 + threadIdx.x;
-  double lfs = 0.0;
-  if (fid < K * 4) {
-    const int e = fid >> 2, f = fid & 3;
-    const int *fm = fmx + (f * 4 + 3) * Nfp;
-    double mv2 = 0.0, mc2 = 0.0;
-    for (int j = 0; j < Nfp; j++) {
-      const size_t idx = (size_t)e * Np + fm[j];
-      if (PHYS == PHYS_EULER) {
-        const double u1 = u[idx], u2 = u[KNp + idx], u3 = u[2 * KNp + idx], u4 = u[3 * KNp + idx],
-                     u5 = u[4 * KNp + idx];
-        const double vx = u2 / u1, vy = u3 / u1, vz = u4 / u1;
-        const double v2 = vx * vx + vy * vy + vz * vz;
-        const double pres = (gamma - 1.0) * (u5 - 0.5 * u1 * v2);
-        const double c2 = fabs(gamma * pres / u1);
-        mv2 = v2 > mv2 ? v2 : mv2;
-        mc2 = c2 > mc2 ? c2 : mc2;
-      } else {
-        const double vx = vel[idx], vy = vel[KNp + idx], vz = vel[2 * KNp + idx];
-        const double v2 = vx * vx + vy * vy + vz * vz;
-        mv2 = v2 > mv2 ? v2 : mv2;
-      }
-    }
-    const double lam = PHYS == PHYS_EULER ? sqrt(mv2) + sqrt(mc2) : sqrt(mv2);
-    maxv[fid] = lam;
-    lfs = lam * geo[(size_t)e * GEO_STRIDE + 9 + f * 4 + 3];
-  }
-  // Block maximum of lambda*Fscale with its face index (smallest on ties).
-  red[threadIdx.x] = lfs;
-  red_i[threadIdx.x] = fid;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      const double o = red[threadIdx.x + s];
-      if (o > red[threadIdx.x] || (o == red[threadIdx.x] && red_i[threadIdx.x + s] < red_i[threadIdx.x])) {
-        red[threadIdx.x] = o;
-        red_i[threadIdx.x] = red_i[threadIdx.x + s];
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = red[0];
-    partial_idx[blockIdx.x] = red_i[0];
-  }
 }
 
 // --------------------------------------------------------------------------

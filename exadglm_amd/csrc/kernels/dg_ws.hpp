@@ -708,8 +708,9 @@ __global__ __launch_bounds__(WsShape<N>::NT) void stage_ws_kernel(StageArgs a, c
 // --------------------------------------------------------------------------
 // LDS-staged wave speed: a block converts the node values of WE elements to
 // |v|^2 and |c|^2 with coalesced loads, then one thread per face takes the
-// maxima over the face nodes.  Same arithmetic and tie rule as
-// wavespeed_kernel.
+// maxima over the face nodes (calc_maxv, euler_dglm.cpp:6-50,
+// advec_dglm.cpp:6-35); the block keeps the largest lambda*Fscale with its
+// face index (smallest index on ties).
 // --------------------------------------------------------------------------
 __host__ __device__ constexpr int wavespeed_elems(int Np) { return Np <= 35 ? 64 : (Np <= 84 ? 32 : 16); }
 
@@ -765,6 +766,9 @@ __global__ __launch_bounds__(256) void wavespeed_lds_kernel(int K, const double 
     const double lam = PHYS == PHYS_EULER ? sqrt(mv2) + sqrt(mc2) : sqrt(mv2);
     maxv[fid] = lam;
     lfs = lam * geo[(size_t)(e0 + e) * GEO_STRIDE + 9 + f * 4 + 3];
+    // A non-finite state (no maximum over a NaN) reaches dt as an infinite
+    // wave speed: dt = 0 is flagged as bad on the device.
+    if (lfs != lfs) lfs = INFINITY;
   }
   red[tid] = lfs;
   red_i[tid] = tid < ne * 4 ? fid : 0x7fffffff;

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -38,6 +39,10 @@ bool launch_ws_dirichlet(int N, int mode, int grid, hipStream_t stream, const St
 
 namespace {
 thread_local std::string g_err;
+
+// Allocation faults requested by dglm_set_alloc_faults (a testing aid: the
+// next n device allocations fail as if the device were out of memory).
+std::atomic<int> g_alloc_faults{0};
 
 int fail(int code, const std::string &msg) {
   g_err = msg;
@@ -107,6 +112,12 @@ struct dglm_ctx {
 
 extern "C" const char *dglm_last_error(void) { return g_err.c_str(); }
 extern "C" int dglm_version(void) { return 100; }
+
+extern "C" int dglm_set_alloc_faults(int n) {
+  if (n < 0) return fail(DGLM_ERR_ARG, "dglm_set_alloc_faults: n must be >= 0");
+  g_alloc_faults.store(n);
+  return DGLM_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Reference-element tables
@@ -400,9 +411,19 @@ StageArgs stage_args(dglm_ctx *c, int s) {
 
 void free_tape(struct dglm_adj_tape *T);
 
+hipError_t device_malloc(void **p, size_t bytes) {
+  int n = g_alloc_faults.load();
+  while (n > 0)
+    if (g_alloc_faults.compare_exchange_weak(n, n - 1)) {
+      *p = nullptr;
+      return hipErrorOutOfMemory;
+    }
+  return hipMalloc(p, bytes);
+}
+
 template <typename T>
 int dev_alloc(T **p, size_t n, const char *what) {
-  hipError_t e = hipMalloc((void **)p, n * sizeof(T) + 16);
+  hipError_t e = device_malloc((void **)p, n * sizeof(T) + 16);
   if (e != hipSuccess) return fail(DGLM_ERR_HIP, std::string("hipMalloc(") + what + "): " + hipGetErrorString(e));
   return DGLM_OK;
 }
@@ -1425,9 +1446,18 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
   const int nck = (nsteps + every - 1) / every;
   // (re)allocate the recording buffers when the shape changes
   if (T->nsteps != nsteps || T->every != every) {
-    void *old[] = {T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2};
+    // The tape is empty until every buffer of the new shape exists: a failed
+    // allocation leaves nsteps = 0, which dglm_adjoint_backward refuses, and a
+    // retry with any shape reallocates.  Kept segments go first (the largest
+    // buffers), so the new shape's buffers do not compete with them.
+    T->nsteps = T->every = T->nck = 0;
+    T->nkeep = -1;
+    void *old[] = {T->keep, T->keep_lam, T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip,
+                   T->ck_step, T->scal2};
     for (void *q : old)
       if (q) HIP_TRY(hipFree(q));
+    T->keep = T->keep_lam = nullptr;
+    T->keep_bytes = 0;
     T->ckpt = T->ck_t = T->dt = T->seg = T->lam = T->kscr = nullptr;
     T->fstar = T->clip = T->ck_step = nullptr;
     T->scal2 = nullptr;
@@ -1441,11 +1471,6 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
     T->nsteps = nsteps;
     T->every = every;
     T->nck = nck;
-    if (T->keep) HIP_TRY(hipFree(T->keep));
-    if (T->keep_lam) HIP_TRY(hipFree(T->keep_lam));
-    T->keep = T->keep_lam = nullptr;
-    T->keep_bytes = 0;  // T->bytes is recomputed below without them
-    T->nkeep = -1;      // sized below
     T->bytes = (2 * nst + (size_t)c->nvar * c->K * c->NFF + (size_t)c->K * 4 + T->nblk_a) * sizeof(double) +
                (nst * (nck + 5 * (size_t)every + 2) + (size_t)c->K * 4 * every) * sizeof(double);
   }
@@ -1467,8 +1492,8 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
     if (nk > nck) nk = nck;
     T->nkeep = (int)nk;
     if (T->nkeep > 0) {
-      if (hipMalloc((void **)&T->keep, seg_n * T->nkeep * sizeof(double) + 16) != hipSuccess ||
-          hipMalloc((void **)&T->keep_lam, lam_n * T->nkeep * sizeof(double) + 16) != hipSuccess) {
+      if (device_malloc((void **)&T->keep, seg_n * T->nkeep * sizeof(double) + 16) != hipSuccess ||
+          device_malloc((void **)&T->keep_lam, lam_n * T->nkeep * sizeof(double) + 16) != hipSuccess) {
         // the budget is an estimate: without the memory, keep nothing and
         // recompute every segment (clear the allocation error first)
         (void)hipGetLastError();
@@ -1609,6 +1634,18 @@ extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *
   const hipMemcpyKind kout = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   HIP_TRY(hipMemcpyAsync(ubar_0, T->ubar, nst * sizeof(double), kout, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  // The kept segments have served their sweep: return their memory (the
+  // largest part of the tape) to the device.  A repeated sweep of the same
+  // recording recomputes every segment from its checkpoint; the next
+  // recording sizes the kept segments anew.
+  if (T->keep || T->keep_lam) {
+    if (T->keep) HIP_TRY(hipFree(T->keep));
+    if (T->keep_lam) HIP_TRY(hipFree(T->keep_lam));
+    T->keep = T->keep_lam = nullptr;
+    T->bytes -= T->keep_bytes;
+    T->keep_bytes = 0;
+    T->nkeep = -1;
+  }
   return DGLM_OK;
 }
 

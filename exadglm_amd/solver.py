@@ -54,6 +54,7 @@ class Solver:
         K_, Np_, Nfp_, nv_ = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         A.check(L.dglm_sizes(self._ctx, C.byref(K_), C.byref(Np_), C.byref(Nfp_), C.byref(nv_)))
         self.K, self.Np, self.Nfp, self.nvar = K_.value, Np_.value, Nfp_.value, nv_.value
+        self.recording = 0  # bumped by every adjoint_record (the tape holds the latest one)
 
     @property
     def shape(self):
@@ -205,6 +206,7 @@ class Solver:
         steps (default ~sqrt(nsteps))."""
         if every is None:
             every = max(1, int(round(math.sqrt(nsteps))))
+        self.recording += 1  # a failed recording also invalidates the previous tape
         A.check(A.lib().dglm_adjoint_record(self._ctx, nsteps, every))
 
     def adjoint_backward(self, ubar_T):
@@ -222,7 +224,9 @@ class Solver:
 
     def set_adjoint_keep(self, max_segments=-1):
         """Checkpoint segments the recording keeps whole (no recompute in the
-        reverse sweep): -1 = as many as half the free device memory holds, 0 = none."""
+        reverse sweep): -1 = as many as half the free device memory holds, 0 = none.
+        Their memory is held from the recording to the end of the next reverse
+        sweep."""
         A.check(A.lib().dglm_adjoint_set_keep(self._ctx, int(max_segments)))
 
     def tape_bytes(self):

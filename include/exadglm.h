@@ -46,6 +46,12 @@ enum { DGLM_FLUX_LM = 0, DGLM_FLUX_RUSANOV = 1 };
 const char *dglm_last_error(void);
 int dglm_version(void);
 
+/* Testing aid (no reference counterpart): the next n device allocations made
+ * by the library fail as if the device were out of memory, so that callers'
+ * tests can exercise its allocation-failure paths without exhausting HBM.
+ * Process-wide; 0 clears it. */
+int dglm_set_alloc_faults(int n);
+
 /* ---------------------------------------------------------------------------
  * Reference-element tables.  Replaces TetraRef / Operator3DRef / Operator3DFace
  * (dim3/src/python/tetra.py:305-331, operator_3d_ref.py:302-349,
@@ -227,7 +233,8 @@ int dglm_adjoint_rhs(dglm_ctx *ctx, const double *u, const double *rbar, double 
  * every `every` steps (sqrt(nsteps) balances memory and recompute). */
 int dglm_adjoint_record(dglm_ctx *ctx, int nsteps, int every);
 /* Reverse sweep over the recorded steps: ubar_0 = (du_T/du_0)^T ubar_T.
- * device_ptrs != 0: both pointers are device memory ([nvar][K*Np]). */
+ * device_ptrs != 0: both pointers are device memory ([nvar][K*Np]).
+ * DGLM_ERR_STATE when nothing is recorded (also after a failed recording). */
 int dglm_adjoint_backward(dglm_ctx *ctx, const double *ubar_T, double *ubar_0, int device_ptrs);
 /* on = 0 freezes tau and dt at their forward values (the exact gradient of
  * the frozen-coefficient map); default 1 (exact; only Euler depends on it). */
@@ -237,7 +244,10 @@ int dglm_adjoint_set_exact(dglm_ctx *ctx, int on);
  * straight into device memory, and the reverse sweep skips their recompute
  * (the gradient is bitwise the same).  -1 (default): as many as half of the
  * free device memory holds at the recording; 0: none.  Takes effect at the
- * next dglm_adjoint_record.  No reference counterpart (the reference has no
+ * next dglm_adjoint_record.  That memory is held from the recording to the end
+ * of the next dglm_adjoint_backward, which releases it (a second sweep of the
+ * same recording recomputes every segment).  If it cannot be allocated the
+ * recording keeps no segment.  No reference counterpart (the reference has no
  * adjoint). */
 int dglm_adjoint_set_keep(dglm_ctx *ctx, int max_segments);
 /* Device memory of the adjoint buffers and tape, kept segments included. */

@@ -20,6 +20,12 @@
  * with the same floating-point expression order, so that it agrees with the
  * compiled reference to rounding (pinned in tests/test_oracle.py).  Faces
  * with EtoB == 1 take a prescribed ub (the reference's Dirichlet stub).
+ *
+ * Beside the reference's LM flux it has the configurations' own surface
+ * fluxes (flux = 1: Rusanov for Euler, exact upwind for advection), written
+ * from the textbook formulas (calc_flux_rusanov) and pinned by a numpy
+ * restatement of those formulas and by their equality with the LM flux on
+ * continuous traces (tests/test_oracle.py).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -147,6 +153,58 @@ static void calc_fluxLM(const oracle_problem *P, const double *tau, const double
   }
 }
 
+/* Textbook numerical fluxes of the configurations (no reference counterpart;
+ * the reference has only the LM flux).  Euler: Rusanov / local Lax-Friedrichs
+ *   F* = (F(u-).n + F(u+).n)/2 + (lambda/2)(u- - u+),
+ *   lambda = max over the two traces of |v.n| + sqrt(gamma p / rho)
+ * (Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics", 3rd ed.,
+ * sec. 10.5.1); advection: exact upwinding F* = (v.n) u-  if v.n > 0, else
+ * (v.n) u+ (v taken at the interior node, as the LM flux does).  The result is
+ * scaled by Fscale like fluxLM.  u+ is the neighbour's node value, or the
+ * prescribed state gD on a Dirichlet face. */
+static void euler_fn(const double U[5], double nx, double ny, double nz, double G, double Fn[5], double *lam) {
+  const double rho = U[0], vx = U[1] / rho, vy = U[2] / rho, vz = U[3] / rho;
+  const double p = (G - 1) * (U[4] - 0.5 * rho * (vx * vx + vy * vy + vz * vz));
+  /* physical fluxes in x, y, z */
+  const double Fx[5] = {rho * vx, rho * vx * vx + p, rho * vy * vx, rho * vz * vx, (U[4] + p) * vx};
+  const double Fy[5] = {rho * vy, rho * vx * vy, rho * vy * vy + p, rho * vz * vy, (U[4] + p) * vy};
+  const double Fz[5] = {rho * vz, rho * vx * vz, rho * vy * vz, rho * vz * vz + p, (U[4] + p) * vz};
+  for (int v = 0; v < 5; v++) Fn[v] = nx * Fx[v] + ny * Fy[v] + nz * Fz[v];
+  const double vn = nx * vx + ny * vy + nz * vz;
+  *lam = fabs(vn) + sqrt(fabs(G * p / rho));
+}
+
+static void calc_flux_rusanov(const oracle_problem *P, const double *u, double *fluxLM) {
+  const int K = P->K, Np = P->Np, Nfp = P->Nfp, NFF = NFACE * Nfp;
+  const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
+  const double G = P->gamma;
+#pragma omp parallel for
+  for (int fidx = 0; fidx < K * NFACE; fidx++) {
+    const double nx = P->nx[fidx], ny = P->ny[fidx], nz = P->nz[fidx], fs = P->Fscale[fidx];
+    const int ei = fidx / NFACE, fi = fidx % NFACE;
+    const int dir = P->gD && P->EtoB && P->EtoB[fidx] == 1;
+    for (int j = 0; j < Nfp; j++) {
+      const size_t idx = (size_t)fidx * Nfp + j;
+      const size_t iM = (size_t)ei * Np + P->Fmask[fi * Nfp + j], iP = (size_t)P->vmapP[idx];
+      if (P->nvar == 5) {
+        double uM[5], uP[5], fm[5], fp[5], lm, lp;
+        for (int v = 0; v < 5; v++) {
+          uM[v] = u[v * KNp + iM];
+          uP[v] = dir ? P->gD[v * KNF + idx] : u[v * KNp + iP];
+        }
+        euler_fn(uM, nx, ny, nz, G, fm, &lm);
+        euler_fn(uP, nx, ny, nz, G, fp, &lp);
+        const double lam = lm > lp ? lm : lp;
+        for (int v = 0; v < 5; v++) fluxLM[v * KNF + idx] = (0.5 * (fm[v] + fp[v]) + 0.5 * lam * (uM[v] - uP[v])) * fs;
+      } else {
+        const double uM = u[iM], uP = dir ? P->gD[idx] : u[iP];
+        const double vn = nx * P->vx[iM] + ny * P->vy[iM] + nz * P->vz[iM];
+        fluxLM[idx] = (vn > 0 ? vn * uM : vn * uP) * fs;
+      }
+    }
+  }
+}
+
 static void calc_fluxes(const oracle_problem *P, const double *u, double *fR, double *fS, double *fT) {
   const int K = P->K, Np = P->Np;
   const size_t KNp = (size_t)K * Np;
@@ -196,20 +254,48 @@ static void update_rhs(const oracle_problem *P, const double *fR, const double *
                        const double *fLM, double *rhs) {
   const int K = P->K, Np = P->Np, NFF = NFACE * P->Nfp;
   const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
-#pragma omp parallel for
-  for (int ei = 0; ei < K; ei++) {
-    for (int i = 0; i < Np; i++) {
+  /* Operators transposed once, so that the loop over output rows i is
+   * contiguous and vectorises; every output keeps the reference's summation
+   * order over j (acc += WDr r + WDs s + WDt t, then acc -= LIFT l), so the
+   * values are those of the row-by-row loops of the reference. */
+  double *T = malloc(((size_t)3 * Np * Np + (size_t)NFF * Np) * sizeof(double));
+  if (!T) abort();
+  double *Tr = T, *Ts = T + (size_t)Np * Np, *Tt = T + (size_t)2 * Np * Np, *Tl = T + (size_t)3 * Np * Np;
+  for (int i = 0; i < Np; i++) {
+    for (int j = 0; j < Np; j++) {
+      Tr[(size_t)j * Np + i] = P->WDr[i * Np + j];
+      Ts[(size_t)j * Np + i] = P->WDs[i * Np + j];
+      Tt[(size_t)j * Np + i] = P->WDt[i * Np + j];
+    }
+    for (int j = 0; j < NFF; j++) Tl[(size_t)j * Np + i] = P->LIFT[i * NFF + j];
+  }
+#pragma omp parallel
+  {
+    double *acc = malloc((size_t)Np * sizeof(double));
+    if (!acc) abort();
+#pragma omp for
+    for (int ei = 0; ei < K; ei++) {
       for (int v = 0; v < P->nvar; v++) {
         const double *r = fR + v * KNp + (size_t)ei * Np, *s = fS + v * KNp + (size_t)ei * Np,
                      *t = fT + v * KNp + (size_t)ei * Np, *l = fLM + v * KNF + (size_t)ei * NFF;
-        double acc = 0;
-        for (int j = 0; j < Np; j++)
-          acc += P->WDr[i * Np + j] * r[j] + P->WDs[i * Np + j] * s[j] + P->WDt[i * Np + j] * t[j];
-        for (int j = 0; j < NFF; j++) acc -= P->LIFT[i * NFF + j] * l[j];
-        rhs[v * KNp + (size_t)ei * Np + i] = acc;
+        for (int i = 0; i < Np; i++) acc[i] = 0;
+        for (int j = 0; j < Np; j++) {
+          const double rj = r[j], sj = s[j], tj = t[j];
+          const double *a = Tr + (size_t)j * Np, *b = Ts + (size_t)j * Np, *c = Tt + (size_t)j * Np;
+          for (int i = 0; i < Np; i++) acc[i] += a[i] * rj + b[i] * sj + c[i] * tj;
+        }
+        for (int j = 0; j < NFF; j++) {
+          const double lj = l[j];
+          const double *a = Tl + (size_t)j * Np;
+          for (int i = 0; i < Np; i++) acc[i] -= a[i] * lj;
+        }
+        double *o = rhs + v * KNp + (size_t)ei * Np;
+        for (int i = 0; i < Np; i++) o[i] = acc[i];
       }
     }
+    free(acc);
   }
+  free(T);
 }
 
 typedef struct {
@@ -242,8 +328,12 @@ static void work_free(work_t *w) {
 }
 
 static void rhs_core(const oracle_problem *P, work_t *w, const double *u, double *rhs) {
-  update_ub(P, u, w->ub);
-  calc_fluxLM(P, w->tau, w->ub, u, w->fLM);
+  if (P->flux == 1) {
+    calc_flux_rusanov(P, u, w->fLM);
+  } else {
+    update_ub(P, u, w->ub);
+    calc_fluxLM(P, w->tau, w->ub, u, w->fLM);
+  }
   calc_fluxes(P, u, w->fR, w->fS, w->fT);
   update_rhs(P, w->fR, w->fS, w->fT, w->fLM, rhs);
 }
@@ -337,6 +427,7 @@ static void euler_jv(const double U[5], double nx, double ny, double nz, const d
 int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, const double *tau, double *drhs) {
   const int K = P->K, Np = P->Np, Nfp = P->Nfp, NFF = NFACE * Nfp, nv = P->nvar;
   const size_t KNp = (size_t)K * Np, KNF = (size_t)K * NFF;
+  if (P->flux != 0) return -2; /* the tangent-linear is of the LM flux only */
   work_t w;
   if (work_alloc(P, &w)) {
     work_free(&w);

@@ -24,6 +24,11 @@ typedef struct oracle_problem {
    * calc_fluxLM): ub = gD there, [nvar][K*4*Nfp]; NULL for none. */
   const int *EtoB;
   const double *gD;
+  /* Numerical flux: 0 = the reference's LM flux (euler_dglm.cpp:110-183);
+   * 1 = the textbook Rusanov (local Lax-Friedrichs) flux for Euler and the
+   * exact upwind flux for advection, the configurations' own fluxes.  On a
+   * Dirichlet face the prescribed state gD is the exterior trace u+. */
+  int flux;
 } oracle_problem;
 
 /* One RHS at state u ([nvar][K*Np]); maxv, tau [K*4] optional. */
@@ -32,7 +37,8 @@ int oracle_rhs(const oracle_problem *P, const double *u, double *rhs, double *ma
 /* RHS at u with the face penalty tau [K*4] given instead of computed. */
 int oracle_rhs_tau(const oracle_problem *P, const double *u, const double *tau, double *rhs);
 
-/* Tangent-linear RHS with tau frozen: drhs = dR/du(u; tau) du.  tau [K*4] is
+/* Tangent-linear RHS with tau frozen: drhs = dR/du(u; tau) du (LM flux only;
+ * -2 for flux != 0).  tau [K*4] is
  * the penalty to hold (the one of the step start inside a Runge-Kutta step);
  * NULL takes the tau that u itself gives. */
 int oracle_rhs_tl(const oracle_problem *P, const double *u, const double *du, const double *tau, double *drhs);

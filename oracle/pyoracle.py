@@ -25,7 +25,7 @@ class OracleProblem(C.Structure):
                 + [(n, PI) for n in ("vmapP", "vmapF", "Fmask")]
                 + [(n, PD) for n in ("WDr", "WDs", "WDt", "LIFT", "Fscale", "rx", "ry", "rz", "sx", "sy", "sz",
                                      "tx", "ty", "tz", "nx", "ny", "nz", "vx", "vy", "vz")]
-                + [("EtoB", PI), ("gD", PD)])
+                + [("EtoB", PI), ("gD", PD), ("flux", C.c_int)])
 
 
 _lib = None
@@ -51,7 +51,10 @@ def _i(a):
 class Oracle:
     """CPU restatement bound to one problem (datain-style arrays)."""
 
-    def __init__(self, problem, physics, cfl=0.8, gamma=1.4, tau_scale=1.2, tmax=math.inf, velocity=None):
+    def __init__(self, problem, physics, cfl=0.8, gamma=1.4, tau_scale=1.2, tmax=math.inf, velocity=None,
+                 flux="lm"):
+        """flux: 'lm' (the reference's) or 'rusanov' (Rusanov for Euler, exact
+        upwind for advection)."""
         a = problem.arrays
         self._keep = dict(a)
         self.nvar = 5 if physics == "euler" else 1
@@ -59,6 +62,7 @@ class Oracle:
         P = OracleProblem()
         P.N, P.Np, P.Nfp, P.K, P.nvar = problem.N, problem.Np, problem.Nfp, problem.K, self.nvar
         P.cfl, P.gamma, P.tau_scale, P.tmax = cfl, gamma, tau_scale, tmax
+        P.flux = {"lm": 0, "rusanov": 1}[flux]
         for n in ("vmapP", "vmapF", "Fmask"):
             setattr(P, n, _i(a[n]))
         for n in ("WDr", "WDs", "WDt", "LIFT", "Fscale", "rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "nx",
@@ -189,7 +193,10 @@ def rhs_tl(oracle, u, du, tau=None):
     if tau is not None:
         tau = np.ascontiguousarray(tau, dtype=np.float64)
         tp = _d(tau)
-    if lib().oracle_rhs_tl(C.byref(oracle.P), _d(u), _d(du), tp, _d(out)) != 0:
+    rc = lib().oracle_rhs_tl(C.byref(oracle.P), _d(u), _d(du), tp, _d(out))
+    if rc == -2:
+        raise ValueError("the oracle's tangent-linear RHS is of the LM flux only")
+    if rc != 0:
         raise MemoryError("oracle_rhs_tl allocation failed")
     return out
 

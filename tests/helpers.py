@@ -42,3 +42,58 @@ def rel_err_per_var(a, b):
 def velocity_for(pb, v):
     n = pb.K * pb.Np
     return [np.full(n, float(c)) for c in v]
+
+
+def face_traces(pb, u):
+    """Interior and exterior traces [nvar, K, 4, Nfp] of a state (vmapM / vmapP)."""
+    a = pb.arrays
+    K, Np, Nfp = pb.K, pb.Np, pb.Nfp
+    Fm = a["Fmask"].reshape(4, Nfp)
+    nv = u.shape[0]
+    uM = np.stack([u[v].reshape(K, Np)[:, Fm.ravel()].reshape(K, 4, Nfp) for v in range(nv)])
+    uP = np.stack([u[v][a["vmapP"].reshape(K, 4, Nfp)] for v in range(nv)])
+    return uM, uP
+
+
+def textbook_surface_flux(pb, u, physics, flux, tau=None, vel=(1.0, 1.0, 1.0), gamma=1.4, sign=1.0):
+    """Numerical normal flux F* [nvar, K, 4, Nfp] (not scaled by Fscale) from
+    the traces, written from the textbook formulas:
+      'rusanov' (Euler)      (F(u-).n + F(u+).n)/2 + lam/2 (u- - u+),
+                             lam = max(|v-.n| + c-, |v+.n| + c+)
+      'rusanov' (advection)  exact upwinding: (v.n) u- if v.n > 0 else (v.n) u+
+      'lm'                   n.F(ub) - tau (ub - u-), ub = (u- + u+)/2
+    (tau [K, 4] needed for 'lm').  sign = -1 plants a mistake for the tests
+    that show the check is sensitive: the Rusanov dissipation with the wrong
+    sign, the downwind trace for advection."""
+    a = pb.arrays
+    uM, uP = face_traces(pb, u)
+    n = np.stack([a["nx"], a["ny"], a["nz"]])[:, :, :, None]
+    if physics == "advection":
+        vn = sum(n[d] * vel[d] for d in range(3))[None]
+        if flux == "rusanov":
+            return np.where(vn * sign > 0, vn * uM, vn * uP)
+        ub = 0.5 * (uM + uP)
+        return vn * ub - tau[None, :, :, None] * (ub - uM)
+
+    def fn(U):
+        rho = U[0]
+        vel3 = U[1:4] / rho
+        p = (gamma - 1) * (U[4] - 0.5 * rho * (vel3 ** 2).sum(0))
+        vn = (n * vel3).sum(0)
+        F = np.stack([rho * vn, U[1] * vn + n[0] * p, U[2] * vn + n[1] * p, U[3] * vn + n[2] * p, (U[4] + p) * vn])
+        return F, np.abs(vn) + np.sqrt(np.abs(gamma * p / rho))
+
+    if flux == "rusanov":
+        fm, lm = fn(uM)
+        fp, lp = fn(uP)
+        return 0.5 * (fm + fp) + sign * 0.5 * np.maximum(lm, lp) * (uM - uP)
+    ub = 0.5 * (uM + uP)
+    return fn(ub)[0] - tau[None, :, :, None] * (ub - uM)
+
+
+def lift_surface(pb, fstar):
+    """-LIFT (F* Fscale) per element: the surface part of the RHS [nvar, K*Np]."""
+    a = pb.arrays
+    nv = fstar.shape[0]
+    f = (fstar * a["Fscale"][None, :, :, None]).reshape(nv, pb.K, 4 * pb.Nfp)
+    return -(f @ a["LIFT"].T).reshape(nv, -1)

@@ -328,12 +328,16 @@ def test_c5_euler_adjoint_full_size(gpu):
     assert e < 1e-5
 
 
-def test_kept_segments_do_not_change_gradient(gpu):
+@pytest.mark.parametrize("nsteps", [20, 21])
+def test_kept_segments_do_not_change_gradient(gpu, nsteps):
     """Segments kept whole by the recording (no recompute in the reverse sweep)
     give the gradient bitwise: none kept, two kept, and the default (as many
-    as half the free memory holds, here all of them); the tape grows with them
-    and the state after the recording is the same."""
-    _say("kept segments")
+    as half the free memory holds, here all of them), also when the last
+    segment is partial (21 steps, checkpoints every 4); the tape grows with
+    them, the state after the recording is the same, and the reverse sweep
+    releases the kept segments' memory (a second sweep of the same recording
+    recomputes them and gives the same gradient)."""
+    _say(f"kept segments, {nsteps} steps")
     pb, u0 = _euler_setup()
     wts = np.random.default_rng(7).uniform(0.5, 1.5, u0.shape)
     res = {}
@@ -342,10 +346,91 @@ def test_kept_segments_do_not_change_gradient(gpu):
             s.set_adjoint_keep(keep)
             s.set_state(u0)
             s.set_time(0.0, 0)
-            s.adjoint_record(20, 4)
+            s.adjoint_record(nsteps, 4)
             uT = s.get_state()
-            res[keep] = (uT, s.adjoint_backward(wts * uT), s.tape_bytes())
+            tape = s.tape_bytes()
+            g = s.adjoint_backward(wts * uT)
+            res[keep] = (uT, g, tape, s.tape_bytes(), s.adjoint_backward(wts * uT))
     for keep in (2, -1):
         np.testing.assert_array_equal(res[keep][0], res[0][0])
         np.testing.assert_array_equal(res[keep][1], res[0][1])
+        np.testing.assert_array_equal(res[keep][4], res[0][1])
+        assert res[keep][3] == res[0][2]  # kept segments released by the sweep
     assert res[0][2] < res[2][2] < res[-1][2]
+
+
+def test_failed_recording_leaves_no_tape(gpu):
+    """An allocation failure while the recording sizes a new tape shape leaves
+    an empty tape: the reverse sweep refuses it (DGLM_ERR_STATE) instead of
+    reading freed buffers, and a retry records and differentiates normally.
+    A failure to allocate the kept segments falls back to keeping none, with
+    the gradient bitwise unchanged."""
+    from exadglm_amd import _capi as A
+
+    _say("failed recording")
+    pb, u0 = _euler_setup()
+    wts = np.random.default_rng(7).uniform(0.5, 1.5, u0.shape)
+    with Solver(pb, "euler", cfl=0.4) as s:
+        s.set_adjoint_keep(0)
+        s.set_state(u0)
+        s.set_time(0.0, 0)
+        s.adjoint_record(12, 4)
+        g_ref = s.adjoint_backward(wts * s.get_state())
+        # a new shape whose first buffer cannot be allocated
+        s.set_state(u0)
+        s.set_time(0.0, 0)
+        A.check(A.lib().dglm_set_alloc_faults(1))
+        try:
+            with pytest.raises(A.DGLMError) as ei:
+                s.adjoint_record(12, 3)
+            assert ei.value.code == A.DGLM_ERR_HIP
+        finally:
+            A.check(A.lib().dglm_set_alloc_faults(0))
+        with pytest.raises(A.DGLMError) as ei:
+            s.adjoint_backward(wts * u0)
+        assert ei.value.code == A.DGLM_ERR_STATE
+        # retry with the old shape: reallocated and recorded
+        s.set_state(u0)
+        s.set_time(0.0, 0)
+        s.adjoint_record(12, 4)
+        np.testing.assert_array_equal(s.adjoint_backward(wts * s.get_state()), g_ref)
+        # kept segments that cannot be allocated: none kept, same gradient
+        s.set_adjoint_keep(-1)
+        s.set_state(u0)
+        s.set_time(0.0, 0)
+        A.check(A.lib().dglm_set_alloc_faults(1))
+        try:
+            s.adjoint_record(12, 4)
+        finally:
+            A.check(A.lib().dglm_set_alloc_faults(0))
+        tape_none = s.tape_bytes()
+        np.testing.assert_array_equal(s.adjoint_backward(wts * s.get_state()), g_ref)
+        s.set_state(u0)
+        s.set_time(0.0, 0)
+        s.adjoint_record(12, 4)
+        assert s.tape_bytes() > tape_none  # with the memory, segments are kept
+        np.testing.assert_array_equal(s.adjoint_backward(wts * s.get_state()), g_ref)
+
+
+def test_autograd_refuses_a_replaced_recording(gpu):
+    """Two dglm_evolve calls on one solver before backward: the solver's tape
+    holds the second trajectory, so backward through the first raises rather
+    than returning the second trajectory's gradient; with a solver per
+    evolve the chained gradient goes through."""
+    import torch
+
+    from exadglm_amd.autograd import dglm_evolve
+
+    _say("autograd re-record guard")
+    pb, u0 = _euler_setup()
+    dev = torch.device("cuda:0")
+    x = torch.tensor(u0, device=dev, requires_grad=True)
+    with Solver(pb, "euler", cfl=0.4) as s:
+        y = dglm_evolve(dglm_evolve(x, s, 4), s, 4)
+        with pytest.raises(RuntimeError, match="re-recorded"):
+            (y * y).sum().backward()
+    x.grad = None
+    with Solver(pb, "euler", cfl=0.4) as s1, Solver(pb, "euler", cfl=0.4) as s2:
+        y = dglm_evolve(dglm_evolve(x, s1, 4), s2, 4, t0=0.0)
+        (y * y).sum().backward()
+    assert x.grad is not None and torch.isfinite(x.grad).all()

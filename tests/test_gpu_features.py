@@ -122,13 +122,15 @@ def test_rusanov_euler_vortex_runs_and_conserves(gpu):
         assert abs(I1[v] - I0[v]) <= 1e-12 * abs(I0[v])
 
 
+@pytest.mark.parametrize("N", [3, 4])
 @pytest.mark.parametrize("variant", ["simple", "mfma", "mfma_ws"])
-def test_rusanov_euler_surface_term(gpu, variant):
+def test_rusanov_euler_surface_term(gpu, variant, N):
     """Euler with the Rusanov option on a discontinuous state: the RHS
     difference to the LM flux equals -LIFT (F*_Rusanov - F*_LM) Fscale with
-    both fluxes computed from the traces in numpy."""
-    _say(f"rusanov euler surface term {variant}")
-    pb = cube_problem(3, 3)
+    both fluxes computed from the traces in numpy (N = 4 is the order of the
+    benchmark workload and its warp-specialised kernel)."""
+    _say(f"rusanov euler surface term {variant} N={N}")
+    pb = cube_problem(N, 3)
     from helpers import noisy_state
 
     u = noisy_state(pb, "euler", seed=12, eps=0.05)
@@ -214,3 +216,29 @@ def test_warp_specialised_kernel_is_independent_of_element_order(gpu, N, nhex):
     assert np.isfinite(runs[0]).all()
     assert np.array_equal(runs[0], back.reshape(5, -1))
     assert np.array_equal(runs[0], runs[2])
+
+
+@pytest.mark.parametrize("where", ["all", "one node"])
+def test_non_finite_state_is_reported(gpu, where):
+    """A state that turns non-finite is reported by dglm_get_time as
+    DGLM_ERR_STATE: a NaN anywhere reaches the wave speed of some face within
+    a step, the device-side dt becomes 0 and the step is flagged (a host-side
+    read of the device flag; the kernels run normally on NaN data)."""
+    from exadglm_amd import _capi as A
+
+    _say(f"non-finite state ({where})")
+    pb = cube_problem(3, 3)
+    u = smooth_state(pb, "euler", seed=2)
+    if where == "all":
+        u[:] = np.nan
+    else:
+        u[0, 17] = np.nan  # an interior node of element 0
+    with Solver(pb, "euler", cfl=0.4) as s:
+        s.set_state(smooth_state(pb, "euler", seed=2))
+        s.step(2)
+        s.time()  # a finite state steps without complaint
+        s.set_state(u)
+        s.step(3)
+        with pytest.raises(A.DGLMError) as ei:
+            s.time()
+        assert ei.value.code == A.DGLM_ERR_STATE

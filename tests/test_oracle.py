@@ -290,3 +290,68 @@ def test_dirichlet_exact_data_keeps_periodic_accuracy():
         errs[name] = np.linalg.norm(u - ex)
     assert errs["cut"] <= errs["per"]
     assert errs["stale"] > 3 * errs["per"]
+
+
+@pytest.mark.parametrize("N", [3, 4])
+@pytest.mark.parametrize("physics", PHYS)
+def test_oracle_rusanov_matches_textbook(physics, N):
+    """The oracle's flux option (Rusanov for Euler, exact upwind for
+    advection; the configurations' fluxes, which the reference lacks) on a
+    state whose traces jump at every face: the RHS difference to the oracle's
+    LM flux equals -LIFT (F*_rus - F*_lm) Fscale with both numerical fluxes
+    restated in numpy from the textbook formulas.  The LM side is the
+    reference's own flux (pinned against the compiled reference above), so
+    this pins the Rusanov / upwind leg."""
+    from helpers import lift_surface, textbook_surface_flux
+
+    pb = cube_problem(N, 3)
+    vel = (1.0, 0.5, -0.25)
+    if physics == "euler":
+        u = noisy_state(pb, "euler", seed=12 + N, eps=0.05)
+    else:
+        u = np.random.default_rng(8 + N).standard_normal((1, pb.K * pb.Np))
+    r_lm, _, tau = O.Oracle(pb, physics, velocity=vel).rhs(u)
+    r_rus, _, tau2 = O.Oracle(pb, physics, velocity=vel, flux="rusanov").rhs(u)
+    assert np.array_equal(tau, tau2)
+    f_rus = textbook_surface_flux(pb, u, physics, "rusanov", vel=vel)
+    f_lm = textbook_surface_flux(pb, u, physics, "lm", tau=tau, vel=vel)
+    expect = lift_surface(pb, f_rus - f_lm)
+    diff = r_rus - r_lm
+    assert np.abs(diff - expect).max() < 1e-12 * np.abs(expect).max()
+    # the check sees a plausible mistake: the dissipation term with the wrong
+    # sign (Euler) / the downwind trace (advection) misses by far more
+    wrong = textbook_surface_flux(pb, u, physics, "rusanov", vel=vel, sign=-1.0)
+    bad = lift_surface(pb, wrong - f_lm)
+    assert np.abs(diff - bad).max() > 1e-3 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("physics", PHYS)
+def test_oracle_rusanov_equals_lm_on_continuous_traces(physics):
+    """Where u- = u+ (a continuous nodal field: shared face nodes coincide)
+    both fluxes reduce to n.F(u): the oracle RHS are equal to rounding."""
+    pb = cube_problem(3, 3)
+    u = smooth_state(pb, physics, seed=5)
+    r_lm = O.Oracle(pb, physics).rhs(u)[0]
+    r_rus = O.Oracle(pb, physics, flux="rusanov").rhs(u)[0]
+    assert rel_err(r_rus, r_lm) < 1e-13
+
+
+@pytest.mark.parametrize("physics", PHYS)
+def test_oracle_rusanov_conserves(physics):
+    """Flux telescoping with the Rusanov / upwind option: sum_e J_e 1^T M rhs_e
+    vanishes on a periodic mesh for a discontinuous state too."""
+    pb = cube_problem(3, 3)
+    u = noisy_state(pb, physics, seed=9, eps=0.05)
+    rhs, _, _ = O.Oracle(pb, physics, flux="rusanov").rhs(u)
+    w = pb.op_ref.M.sum(axis=0)
+    for v in range(rhs.shape[0]):
+        tot = (pb.mesh.J[:, None] * (rhs[v].reshape(pb.K, pb.Np) * w)).sum()
+        scale = (pb.mesh.J[:, None] * np.abs(rhs[v].reshape(pb.K, pb.Np)) * np.abs(w)).sum()
+        assert abs(tot) < 1e-13 * scale
+
+
+def test_oracle_tangent_linear_refuses_rusanov():
+    pb = cube_problem(2, 2)
+    u = smooth_state(pb, "euler", seed=1)
+    with pytest.raises(ValueError):
+        O.rhs_tl(O.Oracle(pb, "euler", flux="rusanov"), u, u)
