@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libexadglm_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HOST_SRCS = ["host/refelem.cpp", "host/mesh.cpp"]
+HOST_SRCS = ["host/refelem.cpp", "host/mesh.cpp", "host/datain.cpp"]
 HIP_SRCS = ["solver.hip", "ws_dirichlet.hip"]
 
 HIP_FLAGS = [
@@ -36,6 +36,33 @@ def _run(cmd, verbose):
     if r.returncode != 0:
         raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}")
     return r.stdout
+
+
+SELFCHECK_SRCS = ["host/selfcheck.cpp", "host/refelem.cpp", "host/mesh.cpp", "host/datain.cpp"]
+SELFCHECK = os.path.join(BUILD, "selfcheck_sanitized")
+SANITIZE_FLAGS = ["-O1", "-g", "-std=c++17", "-ffp-contract=off", "-fno-omit-frame-pointer",
+                  "-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+
+
+def build_selfcheck(verbose=True):
+    """The host-side C++ (operator tables, mesh, datain reader, connectivity
+    packing) with its self-check driver, under AddressSanitizer and UBSan
+    (g++ only: no device code is sanitized)."""
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = f"{SELFCHECK}.tmp{os.getpid()}"  # concurrent builders never expose a half-written binary
+    _run(["g++", *SANITIZE_FLAGS, *[os.path.join(CSRC, s) for s in SELFCHECK_SRCS], "-o", tmp], verbose)
+    os.replace(tmp, SELFCHECK)
+    return SELFCHECK
+
+
+def run_selfcheck(datain_dir=None, N=None):
+    """Run the sanitized self-check (optionally over a datain directory of
+    order N); returns (exit status, output)."""
+    cmd = [SELFCHECK] + ([datain_dir, str(N)] if datain_dir is not None else [])
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:exitcode=3",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1:exitcode=3")
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env)
+    return r.returncode, r.stdout
 
 
 def build(verbose=True):

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/exadglm.h"
+#include "host/datain.h"
 #include "host/mesh.h"
 #include "host/refelem.h"
 #include "kernels/dg_fp64.hpp"
@@ -528,31 +529,12 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
 
   // Packed connectivity from (vmapF, vmapP): find the neighbour face and the
   // node permutation that reproduces vmapP exactly.
-  std::vector<int> conn((size_t)K * 4), dmap((size_t)K * 4, -1), dface;
-  for (int e = 0; e < K; e++)
-    for (int f = 0; f < 4; f++) {
-      const int o = e * 4 + f;
-      const int vf = m->vmapF[o];
-      if (vf < 0 || vf >= 4 * K) return fail(DGLM_ERR_ARG, "dglm_create: vmapF out of range");
-      const int nb = vf / 4, nf = vf % 4;
-      int found = -1;
-      for (int cde = 0; cde < 4 && found < 0; cde++) {
-        bool ok = true;
-        for (int j = 0; j < Nfp && ok; j++) ok = m->vmapP[(size_t)o * Nfp + j] == nb * Np + fmx[(nf * 4 + cde) * Nfp + j];
-        if (ok) found = cde;
-      }
-      if (found < 0)
-        return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: vmapP of face " + std::to_string(o) +
-                                              " is not a rotation of its neighbour face");
-      if (m->EtoB && m->EtoB[o] == 1) {
-        if (found != 3 || nb != e || nf != f)
-          return fail(DGLM_ERR_ARG, "dglm_create: Dirichlet face " + std::to_string(o) +
-                                        " (EtoB == 1) is not a boundary face (vmapP must be vmapM)");
-        dmap[o] = (int)dface.size();
-        dface.push_back(o);
-      }
-      conn[o] = (nb << 4) | (nf << 2) | found;
-    }
+  std::vector<int> conn, dmap, dface;
+  {
+    std::string err;
+    if (int rc = pack_connectivity(K, Np, Nfp, m->vmapP, m->vmapF, m->EtoB, fmx.data(), conn, dmap, dface, err))
+      return fail(rc, "dglm_create: " + err);
+  }
 
   // Geometry records.
   std::vector<double> geo((size_t)K * GEO_STRIDE);
@@ -747,72 +729,32 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   return DGLM_OK;
 }
 
-namespace {
-template <typename T>
-int read_file(const std::string &path, std::vector<T> &v, size_t n) {
-  std::ifstream f(path, std::ios::binary);
-  if (!f.is_open()) return fail(DGLM_ERR_IO, "cannot open " + path);
-  v.resize(n);
-  f.read((char *)v.data(), n * sizeof(T));
-  if ((size_t)f.gcount() != n * sizeof(T))
-    return fail(DGLM_ERR_IO, path + ": expected " + std::to_string(n * sizeof(T)) + " bytes, got " +
-                                 std::to_string(f.gcount()));
-  return DGLM_OK;
-}
-}  // namespace
-
 extern "C" int dglm_create_from_dir(const dglm_params *p, const char *datain_dir, int matrix_order, dglm_ctx **out) {
   if (!p || !datain_dir || !out) return fail(DGLM_ERR_ARG, "dglm_create_from_dir: null argument");
-  std::string d(datain_dir);
-  if (!d.empty() && d.back() != '/') d += '/';
   if (p->N < 1 || p->N > 14) return fail(DGLM_ERR_ARG, "dglm_create_from_dir: bad N");
-  const int Np = tet_np(p->N), Nfp = tri_np(p->N);
-  std::vector<int> dim;
-  int rc;
-  if ((rc = read_file(d + "dim.bin", dim, 1))) return rc;
-  const int K = dim[0];
-  if (K <= 0) return fail(DGLM_ERR_IO, "dim.bin: non-positive element count");
-  std::vector<int> EtoB, vmapP, vmapF, Fmask;
-  std::vector<double> WDr, WDs, WDt, LIFT, Fscale, mt[9], nx, ny, nz, px, py, pz;
-  const char *mnames[9] = {"rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz"};
-  if ((rc = read_file(d + "EtoB.bin", EtoB, (size_t)K * 4)) || (rc = read_file(d + "vmapP.bin", vmapP, (size_t)K * 4 * Nfp)) ||
-      (rc = read_file(d + "vmapF.bin", vmapF, (size_t)K * 4)) || (rc = read_file(d + "Fmask.bin", Fmask, 4 * Nfp)) ||
-      (rc = read_file(d + "WDr.bin", WDr, (size_t)Np * Np)) || (rc = read_file(d + "WDs.bin", WDs, (size_t)Np * Np)) ||
-      (rc = read_file(d + "WDt.bin", WDt, (size_t)Np * Np)) || (rc = read_file(d + "LIFT.bin", LIFT, (size_t)Np * 4 * Nfp)) ||
-      (rc = read_file(d + "Fscale.bin", Fscale, (size_t)K * 4)) || (rc = read_file(d + "nx.bin", nx, (size_t)K * 4)) ||
-      (rc = read_file(d + "ny.bin", ny, (size_t)K * 4)) || (rc = read_file(d + "nz.bin", nz, (size_t)K * 4)) ||
-      (rc = read_file(d + "px.bin", px, (size_t)K * Np)) || (rc = read_file(d + "py.bin", py, (size_t)K * Np)) ||
-      (rc = read_file(d + "pz.bin", pz, (size_t)K * Np)))
-    return rc;
-  for (int q = 0; q < 9; q++)
-    if ((rc = read_file(d + mnames[q] + ".bin", mt[q], (size_t)K))) return rc;
+  Datain d;
+  std::string err;
+  if (int rc = read_datain(datain_dir, tet_np(p->N), tri_np(p->N), d, err)) return fail(rc, err);
   dglm_mesh m{};
-  m.K = K;
+  m.K = d.K;
   m.matrix_order = matrix_order;
-  m.EtoB = EtoB.data();
-  m.vmapP = vmapP.data();
-  m.vmapF = vmapF.data();
-  m.Fmask = Fmask.data();
-  m.WDr = WDr.data();
-  m.WDs = WDs.data();
-  m.WDt = WDt.data();
-  m.LIFT = LIFT.data();
-  m.Fscale = Fscale.data();
-  m.rx = mt[0].data();
-  m.ry = mt[1].data();
-  m.rz = mt[2].data();
-  m.sx = mt[3].data();
-  m.sy = mt[4].data();
-  m.sz = mt[5].data();
-  m.tx = mt[6].data();
-  m.ty = mt[7].data();
-  m.tz = mt[8].data();
-  m.nx = nx.data();
-  m.ny = ny.data();
-  m.nz = nz.data();
-  m.px = px.data();
-  m.py = py.data();
-  m.pz = pz.data();
+  m.EtoB = d.EtoB.data();
+  m.vmapP = d.vmapP.data();
+  m.vmapF = d.vmapF.data();
+  m.Fmask = d.Fmask.data();
+  m.WDr = d.WDr.data();
+  m.WDs = d.WDs.data();
+  m.WDt = d.WDt.data();
+  m.LIFT = d.LIFT.data();
+  m.Fscale = d.Fscale.data();
+  const double **mt[9] = {&m.rx, &m.ry, &m.rz, &m.sx, &m.sy, &m.sz, &m.tx, &m.ty, &m.tz};
+  for (int q = 0; q < 9; q++) *mt[q] = d.metric[q].data();
+  m.nx = d.nx.data();
+  m.ny = d.ny.data();
+  m.nz = d.nz.data();
+  m.px = d.px.data();
+  m.py = d.py.data();
+  m.pz = d.pz.data();
   return dglm_create(p, &m, out);
 }
 
@@ -1476,9 +1418,11 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
   }
   // Segments kept whole: HBM bought back from the recompute.  The count is the
   // caller's cap (dglm_adjoint_set_keep) or, by default, what half of the free
-  // device memory holds; sized once per tape shape.
+  // device memory holds; sized once per tape shape, and again at every
+  // recording that holds none (the memory may have become free since).
   const size_t seg_n = nst * (5 * (size_t)every + 1), lam_n = (size_t)c->K * 4 * every;
-  if (T->nkeep < 0 || (c->adj_keep_max >= 0 && T->nkeep > c->adj_keep_max)) {
+  if (T->nkeep < 0 || (T->nkeep == 0 && c->adj_keep_max != 0) ||
+      (c->adj_keep_max >= 0 && T->nkeep > c->adj_keep_max)) {
     if (T->keep) HIP_TRY(hipFree(T->keep));
     if (T->keep_lam) HIP_TRY(hipFree(T->keep_lam));
     T->keep = T->keep_lam = nullptr;

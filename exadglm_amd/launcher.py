@@ -114,7 +114,27 @@ class DGLM3DBase:
         return b
 
     def build(self, filename_list=None, use_sanitize=False, verbose=1):
-        A.lib()  # the library is prebuilt; fail loudly if it is missing
+        """The library is prebuilt (fail loudly if it is missing).  With
+        use_sanitize (the reference compiles its CPU solver with
+        -fsanitize=address,undefined, dglm_3d_launcher.py:389-390) the host-side
+        C++ of the library is built under AddressSanitizer and UBSan and run
+        over the operators, generated meshes and this work directory's datain/;
+        a sanitizer report or a datain error raises RuntimeError."""
+        A.lib()
+        if not use_sanitize:
+            return
+        from . import build as B
+
+        B.build_selfcheck(verbose=verbose >= 2)
+        runs = [()]
+        if os.path.isdir(getattr(self, "datain_dir", "") or ""):
+            runs.append((self.datain_dir, self.N))
+        for args in runs:
+            rc, out = B.run_selfcheck(*args)
+            if verbose >= 1:
+                print(out.strip())
+            if rc != 0:
+                raise RuntimeError(f"sanitized host self-check failed (status {rc}):\n{out}")
 
     def set_initial_state(self, u0, velocity=None):
         """Override the reference initial condition: u0 [Nvar][K*Np]."""
@@ -138,22 +158,27 @@ class DGLM3DBase:
         return "euler" if self.Nvar == 5 else "advection"
 
     def _write_u(self, u, tstep, physics):
+        """dataout/u_NNNNNNN.bin (advection) or u1_NNNNNNN.bin (Euler density)
+        as REAL (euler_data.cpp:180-187)."""
         name = ("u_" if physics == "advection" else "u1_") + str(tstep).zfill(7) + ".bin"
-        np.ascontiguousarray(u[0]).tofile(os.path.join(self.dataout_dir, name))
+        np.ascontiguousarray(u[0], dtype=self.param["dtype"]).tofile(os.path.join(self.dataout_dir, name))
 
     def run(self, filename, omp_threads=None, verbose=1):
         """Run to TMAX; returns (last_tstep, avg_dt, elapsed_time) like the reference."""
         physics = self._physics(filename)
-        if self.param["dtype"] != "f8":
-            raise NotImplementedError("dtype 'f4' runs through the mixed-precision modes of Solver")
-        prec = "fp64"
+        # dtype 'f4' (REAL = float in the reference, dglm_3d_launcher.py:138-141,
+        # 154; its GPU path then contracts with cublasSgemmBatched): the f4
+        # datain files are read and widened, the operator contractions run in
+        # FP32 on MFMA (orders 1..6; FP64 above), and dataout is written as f4.
+        # The LSRK state, pressure, wave speed and dt stay FP64.
+        real = np.float32 if self.param["dtype"] == "f4" else np.float64
+        prec = "fp32" if real is np.float32 and self.N <= 6 else "fp64"
         start = time.perf_counter()
         s = Solver(N=self.N, physics=physics, cfl=self.param["CFL"], gamma=self.param["gamma"],
                    tau_scale=self.param["tau_scale"], tmax=float(self.tmax), precision=prec, device=self.devid,
                    datain_dir=self.datain_dir, matrix_order=getattr(self, "matrix_order", "c"))
-        px = np.fromfile(os.path.join(self.datain_dir, "px.bin"))
-        py = np.fromfile(os.path.join(self.datain_dir, "py.bin"))
-        pz = np.fromfile(os.path.join(self.datain_dir, "pz.bin"))
+        px, py, pz = (np.fromfile(os.path.join(self.datain_dir, f"{c}.bin"), dtype=real).astype(np.float64)
+                      for c in ("px", "py", "pz"))
         if self._u0 is not None:
             u0 = self._u0
         elif physics == "advection":

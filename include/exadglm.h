@@ -139,8 +139,11 @@ typedef struct dglm_ctx dglm_ctx;
 
 int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **out);
 /* Reads WORK_DIR/datain/{dim,EtoB,vmapP,vmapF,Fmask,WDr,...,pz}.bin as written
- * by save_to_binaries; a missing or short file is DGLM_ERR_IO (the reference
- * calls std::terminate, utils/src/cpp/io.cpp:7-17). */
+ * by save_to_binaries with dtype 'f8' or 'f4' (REAL = double or float,
+ * dglm_3d_launcher.py:138-141,154; the width follows from the file sizes and
+ * f4 values are widened exactly); a missing file, a file of the wrong size or
+ * f4 files mixed with f8 ones is DGLM_ERR_IO (the reference calls
+ * std::terminate, utils/src/cpp/io.cpp:7-17). */
 int dglm_create_from_dir(const dglm_params *p, const char *datain_dir, int matrix_order, dglm_ctx **out);
 int dglm_destroy(dglm_ctx *ctx);
 
@@ -247,7 +250,7 @@ int dglm_adjoint_set_exact(dglm_ctx *ctx, int on);
  * next dglm_adjoint_record.  That memory is held from the recording to the end
  * of the next dglm_adjoint_backward, which releases it (a second sweep of the
  * same recording recomputes every segment).  If it cannot be allocated the
- * recording keeps no segment.  No reference counterpart (the reference has no
+ * recording keeps no segment (the next recording tries again).  No reference counterpart (the reference has no
  * adjoint). */
 int dglm_adjoint_set_keep(dglm_ctx *ctx, int max_segments);
 /* Device memory of the adjoint buffers and tape, kept segments included. */

@@ -89,3 +89,30 @@ def test_null_arguments():
     assert L.dglm_get_state(None, None) == A.DGLM_ERR_ARG
     assert L.dglm_ref_operators(3, None) == A.DGLM_ERR_ARG
     assert L.dglm_uniform_mesh(1, 2, 2, None, None, None, None, None, None, None, None) == A.DGLM_ERR_ARG
+
+
+REALS = ("WDr", "WDs", "WDt", "LIFT", "Fscale", "rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "nx", "ny",
+         "nz", "px", "py", "pz")
+
+
+def test_create_from_dir_reads_f4_and_refuses_mixed_widths():
+    """datain written with dtype 'f4' (REAL = float) is read; one f8 file
+    among f4 ones, or a file of neither size, is DGLM_ERR_IO naming it.
+    (Without a GPU the accepted directory stops at the first device call.)"""
+    L = A.lib()
+    ctx = C.c_void_p()
+    pb = cube_problem(3, 2)
+    with tempfile.TemporaryDirectory() as d:
+        O.write_datain(pb, d)
+        for n in REALS:
+            np.fromfile(os.path.join(d, n + ".bin")).astype(np.float32).tofile(os.path.join(d, n + ".bin"))
+        rc = L.dglm_create_from_dir(C.byref(params()), d.encode(), 0, C.byref(ctx))
+        assert rc != A.DGLM_ERR_IO, L.dglm_last_error()
+        if rc == A.DGLM_OK:
+            L.dglm_destroy(ctx)
+        np.asarray(pb.arrays["ry"], dtype=np.float64).tofile(os.path.join(d, "ry.bin"))
+        rc = L.dglm_create_from_dir(C.byref(params()), d.encode(), 0, C.byref(ctx))
+        assert rc == A.DGLM_ERR_IO and b"ry.bin" in L.dglm_last_error() and b"f8" in L.dglm_last_error()
+        np.zeros(pb.K * 2 + 1, np.float32).tofile(os.path.join(d, "ry.bin"))
+        rc = L.dglm_create_from_dir(C.byref(params()), d.encode(), 0, C.byref(ctx))
+        assert rc == A.DGLM_ERR_IO and b"ry.bin" in L.dglm_last_error()

@@ -345,6 +345,69 @@ def test_launcher_drop_in(gpu, tmp_path, mesh):
     assert abs(l2 - 2.284502600150407e-05) / 2.284502600150407e-05 < 1e-9
 
 
+@pytest.mark.parametrize("dtype", ["f8", "f4"])
+def test_launcher_euler_notebook_flow(gpu, tmp_path, dtype):
+    """The notebook flow of ex3.euler.21.gpu.single.ipynb (density wave,
+    N=5, 6^3 hexes x 6 = 1296 tets, CFL 0.8, tmax 2) through the
+    launcher-compatible API.  dtype 'f8': 1318 steps and the published L2
+    error 2.429869462976010e-05 (ex3.euler.11.cpu.single.ipynb:221).  dtype
+    'f4' (the reference's REAL = float mode): f4 datain and dataout files, the
+    FP32-MFMA contraction; its final density equals the FP64 oracle run on the
+    same f4-rounded datain within the f4 rounding of the output (relative L2
+    1e-7; FP32 rounding alone is 3e-8 RMS)."""
+    _say(f"launcher Euler notebook flow ({dtype})")
+    from glob import glob
+
+    from exadglm_amd.launcher import DGLM3DSingleGPU
+
+    dg = DGLM3DSingleGPU(None, str(tmp_path / "ex3"), 5, 5, 2,
+                         {"CFL": 0.8, "gamma": 1.4, "tau_scale": 1.2, "interpN": 2, "dtype": dtype}, devid=0)
+    dg.init_mesh({"type": "uniform", "param": [7, 7, 7, 0, 1, 0, 1, 0, 1, {"x": "pbc", "y": "pbc", "z": "pbc"}]},
+                 verbose=0)
+    dg.make_work_dir()
+    dg.save_to_binaries(matrix_order="f", verbose=0)
+    names = [f"euler_nopad_{n}.cu" for n in ("data_host", "data_dev", "dglm", "time", "init", "main")]
+    dg.save_to_header([n for n in names if "main" not in n], verbose=0)
+    dg.link_codes(names, verbose=0)
+    dg.build([n for n in names if "main" in n], verbose=0)
+    tstep, avg_dt, _ = dg.run("euler_nopad_main", verbose=0)
+    real = np.float32 if dtype == "f4" else np.float64
+    files = sorted(glob(dg.dataout_dir + "/*.bin"))
+    assert files[0].endswith("u1_0000000.bin") and files[-1].endswith(f"u1_{tstep:07d}.bin")
+    u = np.fromfile(files[-1], dtype=real)
+    assert u.size == dg.Nelem * dg.Np
+    m = dg.mesh
+    x, y, z = m.high_PX.ravel(), m.high_PY.ravel(), m.high_PZ.ravel()
+    wrap = lambda q: (q + 1) % 1 - 1  # noqa: E731
+    ref = 1 + 0.2 * np.sin(2 * np.pi * (wrap(x - 2) + wrap(y + 0.5 * 2) + wrap(z - 2)))
+    l2 = dg.l2_error(ref, u)
+    _say(f"  {tstep} steps, avg dt {avg_dt:.6f}, L2 {l2:.15e}")
+    if dtype == "f8":
+        assert tstep == 1318
+        assert abs(l2 - 2.429869462976010e-05) / 2.429869462976010e-05 < 1e-9
+        return
+    # the FP64 oracle on the f4 datain, widened exactly as the library reads it
+    d = dg.datain_dir
+    rd = lambda n, dt=real: np.fromfile(os.path.join(d, n + ".bin"), dtype=dt)  # noqa: E731
+    arrays = {n: rd(n, np.int32) for n in ("vmapP", "vmapF", "Fmask", "EtoB")}
+    for n in ("Fscale", "rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "nx", "ny", "nz", "px", "py", "pz"):
+        arrays[n] = rd(n).astype(np.float64)
+    for n in ("WDr", "WDs", "WDt", "LIFT"):
+        arrays[n] = np.ascontiguousarray(rd(n).astype(np.float64).reshape((dg.Np, -1), order="F"))
+
+    class _P:
+        pass
+
+    pb = _P()
+    pb.arrays, pb.K, pb.N, pb.Np, pb.Nfp = arrays, dg.Nelem, dg.N, dg.Np, dg.Nfp
+    u0 = ic_euler_density_wave(arrays["px"], arrays["py"], arrays["pz"])
+    uo, dto, to = O.Oracle(pb, "euler", cfl=0.8, tau_scale=1.2, tmax=2.0).run(u0, tstep)
+    assert abs(to - 2.0) < 1e-12 and (dto > 0).sum() == tstep
+    e = rel_err(u.astype(np.float64), uo[0])
+    _say(f"  f4 density vs FP64 oracle on the f4 datain: {e:.3e}")
+    assert e < 1e-7
+
+
 @pytest.mark.parametrize("N", [2, 4, 6])
 def test_mixed_precision_rhs(gpu, N):
     """FP32-MFMA contraction of the offset-split fluxes: one RHS within FP32
