@@ -354,7 +354,9 @@ def test_launcher_euler_notebook_flow(gpu, tmp_path, dtype):
     'f4' (the reference's REAL = float mode): f4 datain and dataout files, the
     FP32-MFMA contraction; its final density equals the FP64 oracle run on the
     same f4-rounded datain within the f4 rounding of the output (relative L2
-    1e-7; FP32 rounding alone is 3e-8 RMS)."""
+    3e-7: the f4 output rounding is 3e-8 RMS, and the FP32 contraction of
+    the interpolation remainders on this coarse mesh (the wave spans ~8
+    elements) adds ~1e-7 over the 1318 steps)."""
     _say(f"launcher Euler notebook flow ({dtype})")
     from glob import glob
 
@@ -405,7 +407,7 @@ def test_launcher_euler_notebook_flow(gpu, tmp_path, dtype):
     assert abs(to - 2.0) < 1e-12 and (dto > 0).sum() == tstep
     e = rel_err(u.astype(np.float64), uo[0])
     _say(f"  f4 density vs FP64 oracle on the f4 datain: {e:.3e}")
-    assert e < 1e-7
+    assert e < 3e-7
 
 
 @pytest.mark.parametrize("N", [2, 4, 6])
@@ -483,6 +485,42 @@ def test_c3_mixed_precision_full_size(gpu, capsys):
     assert max(errs) < 1e-8, errs
     for v in range(5):
         assert abs(e32[v] - e64[v]) <= 0.05 * e64[v]
+
+
+def _vortex_error(N, nhex, T, precision="fp64"):
+    """Per-variable nodal relative L2 error against the analytic translated
+    vortex after running the vortex workload to time T (Rusanov flux, CFL
+    0.4); rho*w against the momentum norm."""
+    pb = uniform_problem(N, nhex + 1, nhex + 1, nhex + 1, box=vortex_box(nhex))
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    with Solver(pb, "euler", cfl=0.4, flux="rusanov", tmax=T, precision=precision) as s:
+        s.set_state(u0)
+        while not s.time()[3]:
+            s.step(50)
+        u = s.get_state()
+        t = s.time()[0]
+    assert abs(t - T) < 1e-12
+    exact = isentropic_vortex(a["px"], a["py"], a["pz"], t=T)
+    return euler_errors(u, exact)
+
+
+def test_isentropic_vortex_against_analytic(gpu):
+    """The isentropic-vortex initial condition and the FP64 solver together,
+    pinned by the analytic translated vortex (no reference counterpart):
+    N = 4 on 12^3 and 24^3 hexes of the vortex box, to T = 1 (the vortex
+    moves by (1, 1)).  Halving h must cut every variable's error by at least
+    2^3.5 (order N+1 = 5 expected; the density error falls 32x in practice),
+    and at 24^3 the density error is below 3e-5 (the superposed periodic
+    images of the analytic solution are accurate to ~1e-9 there)."""
+    _say("isentropic vortex vs analytic, N=4, 12^3 and 24^3 hexes, T=1")
+    e12 = _vortex_error(4, 12, 1.0)
+    e24 = _vortex_error(4, 24, 1.0)
+    rates = [math.log2(e12[v] / e24[v]) for v in range(5)]
+    _say(f"  12^3 {e12}\n  24^3 {e24}\n  observed orders {rates}")
+    for v in (0, 1, 2, 4):  # rho*w is zero in the exact solution (its error is noise)
+        assert rates[v] > 3.5, (v, rates)
+    assert e24[0] < 3e-5
 
 
 @pytest.mark.parametrize("N", [3, 4])
