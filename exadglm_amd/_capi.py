@@ -79,7 +79,12 @@ EXPORTS = [
     "dglm_adjoint_backward", "dglm_adjoint_set_exact", "dglm_adjoint_set_keep", "dglm_adjoint_tape_bytes",
     "dglm_set_graphs",
     "dglm_dirichlet_faces", "dglm_set_dirichlet", "dglm_set_dirichlet_device", "dglm_set_alloc_faults",
+    "dglm_step_noalloc",
 ]
+
+# Limits of the allocation-free entry point (as the header's #defines).
+DGLM_NOALLOC_MAX_STEPS = 65536
+DGLM_NOALLOC_MAX_BLOCKS = 262144
 
 _lib = None
 

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,7 @@ struct dglm_ctx {
   double *u[2] = {nullptr, nullptr};
   double *k = nullptr, *rhs = nullptr, *vel = nullptr, *geo = nullptr, *op = nullptr, *maxv = nullptr,
          *partial = nullptr, *dt_hist = nullptr;
+  double *scratch = nullptr;  // this device's copy of the allocation-free scratch array
   int dt_hist_cap = 0;
   int *conn = nullptr, *fmx = nullptr, *partial_idx = nullptr;
   DevScalars *scal = nullptr;
@@ -256,10 +258,13 @@ extern "C" int dglm_mesh_spatial_h(int K, const int *EtoV, const double *VXYZ, d
 // ---------------------------------------------------------------------------
 namespace {
 
+// Wave speed of state u into maxv, with the per-block maxima of lambda*Fscale
+// (and their face indices) in partial / partial_idx [nblk_ws] for the dt
+// kernel.
 template <int N, int PHYS>
-void launch_wavespeed(dglm_ctx *c, const double *u, double *maxv) {
+void launch_wavespeed(dglm_ctx *c, const double *u, double *maxv, double *partial, int *partial_idx) {
   hipLaunchKernelGGL((wavespeed_lds_kernel<N, PHYS>), dim3(c->nblk_ws), dim3(256), 0, c->stream, c->K, u, c->vel,
-                     c->geo, c->fmx, maxv, c->partial, c->partial_idx, c->p.gamma);
+                     c->geo, c->fmx, maxv, partial, partial_idx, c->p.gamma);
 }
 
 // True when the MFMA stage kernel is built for this order.
@@ -352,7 +357,7 @@ void launch_stage(dglm_ctx *c, const StageArgs &a) {
   hipLaunchKernelGGL((stage_simple_kernel<N, PHYS, MODE>), dim3(nblk), dim3(256), 0, c->stream, a);
 }
 
-typedef void (*ws_fn)(dglm_ctx *, const double *, double *);
+typedef void (*ws_fn)(dglm_ctx *, const double *, double *, double *, int *);
 typedef void (*st_fn)(dglm_ctx *, const StageArgs &);
 
 template <int PHYS>
@@ -482,6 +487,41 @@ int timer_flush(dglm_ctx *c) {
   return DGLM_OK;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Allocation-free stepping (dglm_step_noalloc; no reference counterpart):
+// its device scratch -- the dt history and the wave-speed block maxima with
+// their face indices -- lives in this fixed array instead of buffers the
+// library allocates.  Each device holds its own copy until the process ends;
+// every call on one device shares it, so calls that overlap on one device
+// race.  Layout in doubles: [0, MAX_STEPS) dt history, [MAX_STEPS, +MAX_BLOCKS)
+// block maxima, then MAX_BLOCKS int32 face indices.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_DOUBLES = (size_t)DGLM_NOALLOC_MAX_STEPS + DGLM_NOALLOC_MAX_BLOCKS + DGLM_NOALLOC_MAX_BLOCKS / 2;
+static_assert(NOALLOC_DOUBLES * sizeof(double) == DGLM_NOALLOC_SCRATCH_BYTES, "scratch size");
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // device address of `scratch` per device ordinal (null: not resolved)
+
+// The device address of `scratch` on device dev, resolved once per device
+// with that device current (the array's name is only a host-side stand-in).
+int scratch_on(int dev, double **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0) return fail(DGLM_ERR_ARG, "scratch: bad device ordinal");
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipSetDevice(dev));
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(DGLM_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DGLM_OK;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -659,6 +699,10 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   if (he != hipSuccess) {
     delete c;
     return fail(DGLM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(he));
+  }
+  if ((rc = scratch_on(c->dev, &c->scratch))) {
+    delete c;
+    return rc;
   }
   const size_t nst = (size_t)c->nvar * K * Np;
   if ((rc = dev_alloc(&c->u[0], nst, "u0")) || (rc = dev_alloc(&c->u[1], nst, "u1")) ||
@@ -887,7 +931,7 @@ extern "C" int dglm_rhs(dglm_ctx *c, const double *u, double *rhs, double *maxv,
     if (rc) return rc;
   }
   HIP_TRY(hipMemcpyAsync(c->u[c->cur], u, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ws(c, c->u[c->cur], c->maxv);
+  ws(c, c->u[c->cur], c->maxv, c->partial, c->partial_idx);
   StageArgs a = stage_args(c, -1);
   st1(c, a);
   HIP_TRY(hipGetLastError());
@@ -908,19 +952,23 @@ extern "C" int dglm_rhs(dglm_ctx *c, const double *u, double *rhs, double *maxv,
 }
 
 namespace {
-// One LSRK4(5) step: wave speed, device-side dt, five fused stages.
-void enqueue_step(dglm_ctx *c, ws_fn ws, st_fn st0, double *dt_hist, int hist_idx) {
+// One LSRK4(5) step: wave speed, device-side dt, five fused stages.  The
+// wave-speed block maxima go through partial / partial_idx [nblk_ws]; dt_hist
+// (device, optional) receives dt at hist_idx.  timed: per-kernel HIP events
+// when the context's timing is on.
+void enqueue_step(dglm_ctx *c, ws_fn ws, st_fn st0, double *dt_hist, int hist_idx, double *partial,
+                  int *partial_idx, bool timed = true) {
   const double cfl_fac = c->p.cfl / (2 * c->p.N + 1);
-  int tk = timer_start(c, 0);
-  ws(c, c->u[c->cur], c->maxv);
+  int tk = timed ? timer_start(c, 0) : -1;
+  ws(c, c->u[c->cur], c->maxv, partial, partial_idx);
   timer_stop(c, tk);
-  tk = timer_start(c, 1);
-  hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
-                     c->nblk_ws, c->scal, cfl_fac, c->p.tmax, dt_hist, (int *)nullptr, (int *)nullptr, hist_idx);
+  tk = timed ? timer_start(c, 1) : -1;
+  hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, partial, partial_idx, c->nblk_ws,
+                     c->scal, cfl_fac, c->p.tmax, dt_hist, (int *)nullptr, (int *)nullptr, hist_idx);
   timer_stop(c, tk);
   for (int s = 0; s < 5; s++) {
     StageArgs a = stage_args(c, s);
-    tk = timer_start(c, 2);
+    tk = timed ? timer_start(c, 2) : -1;
     st0(c, a);
     timer_stop(c, tk);
     c->cur ^= 1;
@@ -941,8 +989,8 @@ int build_graph(dglm_ctx *c, ws_fn ws, st_fn st0) {
   const int start = c->cur;
   hipGraph_t g = nullptr;
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(c, ws, st0, nullptr, 0);
-  enqueue_step(c, ws, st0, nullptr, 0);
+  enqueue_step(c, ws, st0, nullptr, 0, c->partial, c->partial_idx);
+  enqueue_step(c, ws, st0, nullptr, 0, c->partial, c->partial_idx);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   c->cur = start;
   if (e != hipSuccess) return fail(DGLM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -977,7 +1025,7 @@ extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
   const bool graphs = c->use_graph && !dt_hist && !c->timing;
   if (graphs && nsteps >= 3) {
     if (!c->graph[c->cur ^ 1] && !c->graph[c->cur]) {
-      enqueue_step(c, ws, st0, nullptr, 0);
+      enqueue_step(c, ws, st0, nullptr, 0, c->partial, c->partial_idx);
       st++;
     }
     while (nsteps - st >= 2) {
@@ -990,7 +1038,7 @@ extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
     }
   }
   for (; st < nsteps; st++) {
-    enqueue_step(c, ws, st0, dt_hist ? c->dt_hist : nullptr, st);
+    enqueue_step(c, ws, st0, dt_hist ? c->dt_hist : nullptr, st, c->partial, c->partial_idx);
     if (c->ev_class.size() > 8192) {
       int rc = timer_flush(c);
       if (rc) return rc;
@@ -999,6 +1047,36 @@ extern "C" int dglm_step(dglm_ctx *c, int nsteps, double *dt_hist) {
   HIP_TRY(hipGetLastError());
   if (dt_hist) {
     HIP_TRY(hipMemcpyAsync(dt_hist, c->dt_hist, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return DGLM_OK;
+}
+
+extern "C" int dglm_step_noalloc(dglm_ctx *c, int nsteps, double *dt_hist) {
+  if (!c || nsteps < 0) return fail(DGLM_ERR_ARG, "dglm_step_noalloc: bad arguments");
+  if (nsteps > DGLM_NOALLOC_MAX_STEPS)
+    return fail(DGLM_ERR_ARG, "dglm_step_noalloc: nsteps = " + std::to_string(nsteps) +
+                                  " exceeds DGLM_NOALLOC_MAX_STEPS = " + std::to_string(DGLM_NOALLOC_MAX_STEPS) +
+                                  " (step in chunks)");
+  if (c->nblk_ws > DGLM_NOALLOC_MAX_BLOCKS)
+    return fail(DGLM_ERR_ARG, "dglm_step_noalloc: the mesh has " + std::to_string(c->nblk_ws) +
+                                  " wave-speed blocks, more than DGLM_NOALLOC_MAX_BLOCKS = " +
+                                  std::to_string(DGLM_NOALLOC_MAX_BLOCKS));
+  if (int rc = need_dirichlet(c, "dglm_step_noalloc")) return rc;
+  ws_fn ws;
+  st_fn st0, st1;
+  if (!dispatch(c, ws, st0, st1)) return fail(DGLM_ERR_UNSUPPORTED, "dglm_step_noalloc: order not built");
+  HIP_TRY(hipSetDevice(c->dev));
+  // Every scratch entry a step reads is written earlier in the same step
+  // (block maxima by the wave-speed kernel, dt by the dt kernel), so nothing
+  // left by an earlier call is read and nothing needs clearing.
+  double *hist = c->scratch;
+  double *part = c->scratch + DGLM_NOALLOC_MAX_STEPS;
+  int *pidx = (int *)(part + DGLM_NOALLOC_MAX_BLOCKS);
+  for (int st = 0; st < nsteps; st++) enqueue_step(c, ws, st0, dt_hist ? hist : nullptr, st, part, pidx, false);
+  HIP_TRY(hipGetLastError());
+  if (dt_hist) {
+    HIP_TRY(hipMemcpyAsync(dt_hist, hist, (size_t)nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   }
   return DGLM_OK;
@@ -1357,7 +1435,7 @@ extern "C" int dglm_adjoint_rhs(dglm_ctx *c, const double *u, const double *rbar
   else pick_adj<PHYS_ADV>(c->p.N, A, B, L, eb);
   HIP_TRY(hipMemcpyAsync(c->u[c->cur], u, nst * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(T->rbar, rbar, nst * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  ws(c, c->u[c->cur], c->maxv);  // tau at u
+  ws(c, c->u[c->cur], c->maxv, c->partial, c->partial_idx);  // tau at u
   AdjArgs a = adj_args(c);
   a.mode = 1;
   a.u = c->u[c->cur];
@@ -1469,7 +1547,7 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
       double *lam = T->keep_lam + lam_n * (q - q0) + (size_t)c->K * 4 * (n - n0);
       if (n == n0)
         HIP_TRY(hipMemcpyAsync(base, c->u[c->cur], nst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      ws(c, base, lam);
+      ws(c, base, lam, c->partial, c->partial_idx);
       hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
                          c->nblk_ws, c->scal, cfl_fac, c->p.tmax, T->dt, T->fstar, T->clip, n);
       for (int s = 0; s < 5; s++) st0(c, make_stage_args(c, base + nst * s, base + nst * (s + 1), c->k, lam, c->scal, s));
@@ -1478,7 +1556,7 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
                                c->stream));
       continue;
     }
-    ws(c, c->u[c->cur], c->maxv);
+    ws(c, c->u[c->cur], c->maxv, c->partial, c->partial_idx);
     hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
                        c->nblk_ws, c->scal, cfl_fac, c->p.tmax, T->dt, T->fstar, T->clip, n);
     for (int s = 0; s < 5; s++) {
@@ -1535,7 +1613,7 @@ extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *
       for (int n = n0; n < n1; n++) {
         double *base = T->seg + nst * 5 * (size_t)(n - n0);
         double *lam = T->lam + (size_t)c->K * 4 * (n - n0);
-        ws(c, base, lam);
+        ws(c, base, lam, c->partial, c->partial_idx);
         hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1024), 0, c->stream, c->partial, c->partial_idx,
                            c->nblk_ws, T->scal2, cfl_fac, c->p.tmax, (double *)nullptr, (int *)nullptr,
                            (int *)nullptr, 0);

@@ -155,6 +155,13 @@ class Solver:
         A.check(A.lib().dglm_step(self._ctx, nsteps, None))
         return None
 
+    def step_noalloc(self, nsteps=1, return_dt=False):
+        """step() through the allocation-free entry point (no device
+        allocation by the library; scratch in its fixed per-device array)."""
+        dts = np.empty(nsteps) if return_dt else None
+        A.check(A.lib().dglm_step_noalloc(self._ctx, nsteps, A.dptr(dts)))
+        return dts
+
     def set_stage_kernel(self, variant):
         """'auto', 'simple' (LDS-staged FP64), 'mfma' (one workgroup per batch,
         operator rows in registers) or 'mfma_ws' (persistent warp-specialised

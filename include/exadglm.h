@@ -191,6 +191,25 @@ int dglm_rhs(dglm_ctx *ctx, const double *u, double *rhs, double *maxv, double *
  * dt_hist [nsteps] may be NULL.  Steps after t reaches tmax are no-ops. */
 int dglm_step(dglm_ctx *ctx, int nsteps, double *dt_hist);
 
+/* Allocation-free stepping.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Advances nsteps like
+ * dglm_step, with the same kernels and bitwise the same states and dt history
+ * (dt_hist [nsteps], host, may be NULL), but calls no hipMalloc, hipFree or
+ * allocating library routine: its device scratch -- the dt history and the
+ * wave-speed block maxima -- lives in a fixed array of the library
+ * (DGLM_NOALLOC_SCRATCH_BYTES on every device it runs on, held until the
+ * process ends), and the steps are launched one kernel at a time (no graph,
+ * no timing events).  nsteps > DGLM_NOALLOC_MAX_STEPS, or a mesh with more
+ * than DGLM_NOALLOC_MAX_BLOCKS wave-speed blocks (K/64 at N <= 4, K/32 at
+ * N = 5, 6, K/16 above), is DGLM_ERR_ARG.  All calls on one device share that
+ * array: calls that overlap on one device (several threads or contexts at
+ * once) race. */
+#define DGLM_NOALLOC_MAX_STEPS 65536
+#define DGLM_NOALLOC_MAX_BLOCKS 262144
+#define DGLM_NOALLOC_SCRATCH_BYTES (8 * (DGLM_NOALLOC_MAX_STEPS + DGLM_NOALLOC_MAX_BLOCKS + DGLM_NOALLOC_MAX_BLOCKS / 2))
+int dglm_step_noalloc(dglm_ctx *ctx, int nsteps, double *dt_hist);
+
 /* Conserved integrals sum_e J_e 1^T M u_e per variable, FP64 on the device
  * with a fixed-order reduction; J [K], M [Np*Np]; out [nvar]. */
 int dglm_integrals(dglm_ctx *ctx, const double *J, const double *M, double *out);
