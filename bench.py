@@ -149,8 +149,9 @@ def mixed_precision_leg(order=6, nhex=24, steps=490, warmup=10):
         "workload": f"euler_isentropic_vortex_N{order}_{nhex}cube_hexes_x6_K{pb.K}_rusanov_{warmup + steps}steps",
         "fp64": {"value": dof_stage / r64["wall"], "ms_per_step": r64["wall"] / steps * 1e3, "kernel": r64["kernel"]},
         "mixed": {"value": dof_stage / r32["wall"], "ms_per_step": r32["wall"] / steps * 1e3, "kernel": r32["kernel"],
-                  "contraction": "FP32 MFMA on per-block FP64-offset fluxes, FP32 accumulation; state, pressure, "
-                                 "wave speed, offsets and sums FP64"},
+                  "contraction": "FP32 MFMA on the remainders of per-block least-squares affine fits of the FP64 "
+                                 "fluxes, FP32 accumulation; the fits and their contraction (FP64 MFMA), state, "
+                                 "pressure, wave speed and sums FP64"},
         "speedup": r64["wall"] / r32["wall"],
         "rel_l2_mixed_vs_fp64": per_var_err(r32["u"], r64["u"]),
         "rel_l2_fp64_vs_analytic": per_var_err(r64["u"], exact),

@@ -29,6 +29,7 @@ DGLM_STAGE_AUTO = 0
 DGLM_STAGE_SIMPLE = 1
 DGLM_STAGE_MFMA = 2
 DGLM_STAGE_MFMA_WS = 3
+DGLM_STAGE_MFMA_MX = 4
 
 _ERRNAMES = {
     DGLM_ERR_ARG: "DGLM_ERR_ARG",

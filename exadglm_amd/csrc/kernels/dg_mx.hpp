@@ -1,0 +1,496 @@
+// Mixed-precision Euler stage kernel (gfx950): the operator contraction on
+// FP32 MFMA, applied to what a per-block least-squares affine fit leaves of
+// each flux block; the fit is contracted exactly in FP64.
+//
+// The stage contraction  rhs[v][e][i] = sum_k A[i][k] B_v[k][e]  (update_rhs,
+// dim3/src/cpp/euler_dglm.cpp:254-302; A = [WDr|WDs|WDt|-LIFT], B_v the
+// stacked volume fluxes and surface fluxes of variable v) of a smooth field
+// is a small difference of large volume and surface terms, so a plain FP32
+// contraction loses ~1e-7 relative.  Each flux block b (the three volume
+// directions over the element's nodes, each face over its nodes) is written
+// as an affine function V_b c_b plus a remainder:
+//   B_b = V_b c_b + R_b,
+//   rhs = A R (FP32 MFMA, FP32 accumulation) + (A V) c (FP64 MFMA),
+// with c_b = P_b^T B_b the least-squares fit (FP64; V, P from the reference
+// nodes, the tables of affine_fit_tables).  The identity holds for any c, so
+// the split changes rounding only: only R is rounded to FP32.  The LSRK state
+// and update, pressure, wave speed, fluxes, fits and the (A V) c term stay
+// FP64.
+//
+// Organisation: one persistent workgroup per CU of eight waves: RT =
+// ceil(Np/16) contraction waves and 8 - RT helpers.  Contraction wave t keeps
+// the FP32 A-fragments of rows 16t..16t+15 (all k-steps) and its rows of
+// (A V) in registers for the whole kernel.  Per 16-element batch (the 16 MFMA
+// columns), with the node values of the batch staged in LDS:
+//   1. node values, geometry, connectivity and tau into LDS;
+//   2. the surface flux of all five variables at every face-node pair, once,
+//      its face fits, and the remainders rounded into an FP32 tile per
+//      variable (eight threads per face);
+//   3. variable 0's volume pass: volume fluxes, their fits, the remainder tile;
+//   4. per variable v: the contraction waves run the MFMA contraction over
+//      [volume | surface] k-steps, the fit term (A V) c on FP64 MFMA and the
+//      LSRK update; then every wave takes two elements of the volume pass of
+//      variable v + 1 into the other remainder buffer; one barrier per
+//      variable.
+// Every wave takes part in the VALU passes (two elements of a volume pass
+// each), so their latency is spread over eight waves.
+//
+// No Dirichlet faces and no advection here: those take the generic MFMA
+// kernel (dg_mfma.hpp).
+#pragma once
+#include <vector>
+
+#include "dg_mfma.hpp"
+
+namespace dglm {
+
+template <int N>
+struct MxShape {
+  using O = Ord<N>;
+  static constexpr int Np = O::Np, Nfp = O::Nfp, NFF = O::NFF, KD = O::KD;
+  static constexpr int RT = (Np + 15) / 16;     // row tiles = contraction waves
+  static constexpr int NW = 8;                  // waves: RT contraction waves + 8 - RT helpers
+  static constexpr int NT = 64 * NW;
+  static constexpr int KSV = (3 * Np + 3) / 4;  // volume k-steps (rows 3Np..4KSV-1 are zero)
+  static constexpr int KV4 = 4 * KSV;
+  static constexpr int KS = KSV + Nfp;  // + surface k-steps (NFF = 4 Nfp rows)
+  static constexpr int EB = 16;
+  static constexpr int NC = 24;  // fit coefficients per (variable, element): 3 x 4 volume + 4 x 3 face
+  static constexpr int UST = Np + 1;  // element stride of the staged node values (odd: a 16-element
+                                      // column spreads over the LDS banks)
+};
+
+// Device tables of the kernel (host-built, one buffer):
+//   afr  [RT][KS][64] float    A-fragments, k order [volume rows padded to KV4 | surface rows]
+//   avf  [RT][NC/4][64] double FP64 A-fragments of (A V), row i's weight of each
+//                              fit coefficient, rows permuted so that the FP64
+//                              MFMA's D layout (row 4r + l/16) matches the FP32
+//                              one (row 4(l/16) + r): fragment row 4r + q holds
+//                              row 4q + r of the tile
+//   pv, vv [Np][4] double      volume fit weights and basis (the three
+//                              directions share the node set)
+//   pf, vf [NFF][4] double     face fit weights and basis (q = 3 unused)
+template <int N>
+struct MxTables {
+  using S = MxShape<N>;
+  static constexpr size_t afr_off = 0;
+  static constexpr size_t afr_bytes = sizeof(float) * S::RT * S::KS * 64;
+  static constexpr size_t avf_off = (afr_bytes + 255) / 256 * 256;
+  static constexpr size_t pv_off = avf_off + sizeof(double) * S::RT * S::NC * 16;
+  static constexpr size_t vv_off = pv_off + sizeof(double) * S::Np * 4;
+  static constexpr size_t pf_off = vv_off + sizeof(double) * S::Np * 4;
+  static constexpr size_t vf_off = pf_off + sizeof(double) * S::NFF * 4;
+  static constexpr size_t bytes = vf_off + sizeof(double) * S::NFF * 4;
+};
+
+// Host-side: fill the tables of order N from the stacked operator opT
+// [KD][Np] (op[k][i] = A[i][k], -LIFT included) and the affine-fit tables
+// fitV, fitP [KD][4], fitAV [Np][NBLK][4] of affine_fit_tables.
+template <int N>
+inline void mx_build_tables(const double *opT, const double *fitV, const double *fitP, const double *fitAV,
+                            unsigned char *buf) {
+  using S = MxShape<N>;
+  constexpr int Np = S::Np;
+  using T = MxTables<N>;
+  float *afr = (float *)(buf + T::afr_off);
+  double *avf = (double *)(buf + T::avf_off), *pv = (double *)(buf + T::pv_off), *vv = (double *)(buf + T::vv_off);
+  double *pf = (double *)(buf + T::pf_off), *vf = (double *)(buf + T::vf_off);
+  for (int j = 0; j < Np; j++)
+    for (int q = 0; q < 4; q++) {
+      pv[j * 4 + q] = fitP[j * 4 + q];  // block 0 (rows 0..Np-1); blocks 1, 2 are the same nodes
+      vv[j * 4 + q] = fitV[j * 4 + q];
+    }
+  for (int m = 0; m < S::NFF; m++)
+    for (int q = 0; q < 4; q++) {
+      pf[m * 4 + q] = fitP[(3 * Np + m) * 4 + q];
+      vf[m * 4 + q] = fitV[(3 * Np + m) * 4 + q];
+    }
+  // A-fragments in the kernel's k order
+  auto kmap = [&](int kk) -> int {  // kernel row -> stacked row (-1: zero)
+    if (kk < S::KV4) return kk < 3 * Np ? kk : -1;
+    const int m = kk - S::KV4;
+    return m < S::NFF ? 3 * Np + m : -1;
+  };
+  for (int w = 0; w < S::RT; w++)
+    for (int ks = 0; ks < S::KS; ks++)
+      for (int l = 0; l < 64; l++) {
+        const int i = 16 * w + (l & 15), k = kmap(4 * ks + (l >> 4));
+        afr[((size_t)w * S::KS + ks) * 64 + l] = (i < Np && k >= 0) ? (float)opT[(size_t)k * Np + i] : 0.0f;
+      }
+  // (A V): coefficient q' = d*4 + q (volume direction d) or 12 + f*3 + q (face f)
+  auto av = [&](int i, int qq) {
+    if (qq < 12) return fitAV[((size_t)i * NBLK + qq / 4) * 4 + qq % 4];
+    const int f = (qq - 12) / 3, q = (qq - 12) % 3;
+    return fitAV[((size_t)i * NBLK + 3 + f) * 4 + q];
+  };
+  for (int w = 0; w < S::RT; w++)
+    for (int ks = 0; ks < S::NC / 4; ks++)
+      for (int l = 0; l < 64; l++) {
+        const int rho = l & 15, i = 16 * w + 4 * (rho & 3) + (rho >> 2), qq = 4 * ks + (l >> 4);
+        avf[((size_t)w * (S::NC / 4) + ks) * 64 + l] = i < Np ? av(i, qq) : 0.0;
+      }
+}
+
+// Workgroup barrier over LDS: waits for this wave's LDS (and scalar memory)
+// operations only, not for its global loads and stores, so those stay in
+// flight across it (no wave of the workgroup reads global memory another wave
+// writes during the launch: the stage reads u_in and writes k and u_out).  A
+// plain __syncthreads would also drain every outstanding global access.
+__device__ inline void mx_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Surface flux of the five variables times Fscale at one face-node pair:
+// LM (the reference's, euler_dglm.cpp:110-183) or Rusanov.  g = (nx, ny, nz,
+// Fscale) of the face.
+__device__ inline void mx_surface_flux5(const double uM[5], const double uP[5], const double *g, double tau,
+                                        int flux, double gamma, double fl[5]) {
+  if (flux == 1) {
+    euler_rusanov5(uM, uP, g, gamma, fl);
+#pragma unroll
+    for (int v = 0; v < 5; v++) fl[v] *= g[3];
+    return;
+  }
+  double ub[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + uP[v]);
+  const double inv = 1.0 / ub[0];
+  const double p = (gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
+  const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
+  double nF[5];
+  nF[0] = g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3];
+  nF[1] = ub[1] * vn + g[0] * p;
+  nF[2] = ub[2] * vn + g[1] * p;
+  nF[3] = ub[3] * vn + g[2] * p;
+  nF[4] = (ub[4] + p) * vn;
+#pragma unroll
+  for (int v = 0; v < 5; v++) fl[v] = (nF[v] - tau * (ub[v] - uM[v])) * g[3];
+}
+
+// Volume fluxes (fR, fS, fT) of variable v at one node: g_d . F with the
+// contravariant velocity V_d = g_d . v (no array indexed by v, so nothing
+// here goes through private memory).
+__device__ inline void mx_volume_flux1(int v, const double u[5], const double *gm, double gamma, double f[3]) {
+  const double r = u[0], mx = u[1], my = u[2], mz = u[3], E = u[4];
+  const double inv = 1.0 / r;
+  const double p = (gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
+  const double vx = mx * inv, vy = my * inv, vz = mz * inv;
+  const double mv = v == 1 ? mx : (v == 2 ? my : mz);  // momentum component of v = 1..3
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double g0 = gm[3 * d], g1 = gm[3 * d + 1], g2 = gm[3 * d + 2];
+    const double V = g0 * vx + g1 * vy + g2 * vz;
+    const double gv = v == 1 ? g0 : (v == 2 ? g1 : g2);
+    if (v == 0)
+      f[d] = g0 * mx + g1 * my + g2 * mz;
+    else if (v == 4)
+      f[d] = (E + p) * V;
+    else
+      f[d] = mv * V + gv * p;
+  }
+}
+
+// Volume pass of wave w for variable v: the volume fluxes at every node of
+// elements 2w, 2w+1 of the batch, their least-squares fits c (FP64, into cv)
+// and the FP32 remainders into vr [KV4][EB].  Lane (element el, group sl)
+// takes nodes j = sl + 32x; the fit sums meet across the 32 groups in
+// registers.
+template <int N>
+__device__ inline void mx_volume_pass(int v, int w, int lane, int ne, const double *s_u, const double *s_geo,
+                                      const double *pv, const double *vv, double gamma, double *cv, float *vr) {
+  using S = MxShape<N>;
+  constexpr int Np = S::Np, EB = S::EB, UST = S::UST, JX = (Np + 31) / 32;
+  const int el = 2 * w + (lane >> 5), sl = lane & 31;
+  const double *gm = &s_geo[el * GEO_STRIDE];
+  double f[JX][3];
+  double c[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) c[q] = 0.0;
+#pragma unroll
+  for (int x = 0; x < JX; x++) {
+    const int j = sl + 32 * x;
+    f[x][0] = f[x][1] = f[x][2] = 0.0;
+    if (j < Np) {
+      double u[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) u[q] = s_u[q * EB * UST + el * UST + j];
+      mx_volume_flux1(v, u, gm, gamma, f[x]);
+      const double *P = &pv[j * 4];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) c[d * 4 + q] += P[q] * f[x][d];
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1)
+#pragma unroll
+    for (int q = 0; q < 12; q++) c[q] += __shfl_xor(c[q], o, 64);
+  if (sl == 0)
+#pragma unroll
+    for (int q = 0; q < 12; q++) cv[q * EB + el] = c[q];
+#pragma unroll
+  for (int x = 0; x < JX; x++) {
+    const int j = sl + 32 * x;
+    if (j < Np) {
+      const double *V = &vv[j * 4];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const double fit = V[0] * c[d * 4] + V[1] * c[d * 4 + 1] + V[2] * c[d * 4 + 2] + V[3] * c[d * 4 + 3];
+        vr[(d * Np + j) * EB + el] = el < ne ? (float)(f[x][d] - fit) : 0.0f;
+      }
+    }
+  }
+}
+
+//   MODE 0: LSRK update; MODE 1: write the RHS only.
+// order: batch visiting order (null: mesh order); the grid is split into ngrp
+// groups (blockIdx % ngrp) that walk contiguous stretches of it.
+template <int N, int MODE>
+__global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, const unsigned char *__restrict__ tab,
+                                                                  const int *__restrict__ order, int nbatch,
+                                                                  int ngrp) {
+  using S = MxShape<N>;
+  using TB = MxTables<N>;
+  constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, KS = S::KS, KSV = S::KSV, KV4 = S::KV4;
+  constexpr int NT = S::NT, RT = S::RT, EB = S::EB, NC = S::NC, UST = S::UST;
+  constexpr int JF = (Nfp + 7) / 8;              // face nodes per thread in the surface pass
+  constexpr int NLD = (EB * Np + NT - 1) / NT;   // node values per thread and variable in the staging
+  static_assert(S::NW * 2 == EB && RT <= S::NW, "a volume pass takes two elements per wave");
+  static_assert(NT == 4 * EB * 8, "the surface pass takes a thread per (face, group of eight)");
+  const float *afrag = (const float *)(tab + TB::afr_off);
+  const double *avfrag = (const double *)(tab + TB::avf_off);
+
+  __shared__ double s_u[5][EB * UST];     // node values of the batch [v][e*UST + j]
+  __shared__ float s_vr[2][KV4 * EB];     // volume remainders of two variables [k][e]
+  __shared__ float s_sr[5][NFF * EB];     // surface remainders of all variables [v][m][e]
+  __shared__ double s_c[5][NC * EB];      // fit coefficients [v][q'][e]
+  __shared__ double s_tabv[2][Np * 4];    // fit weights and basis of the volume blocks
+  __shared__ double s_tabf[2][NFF * 4];   // ... and of the faces
+  __shared__ double s_geo[EB * GEO_STRIDE];
+  __shared__ double s_tau[EB * 4];
+  __shared__ int s_conn[EB * 4];
+  __shared__ int s_fmx[16 * Nfp];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: role branches are scalar
+  const bool cw = w < RT;                                   // contraction wave (tile w) or helper
+  const int K = a.K;
+  const size_t KNp = (size_t)K * Np;
+
+  if (MODE == 0 && !a.scal->active) {
+    // Past tmax: the stage is the identity on u (k untouched).
+    for (size_t i = (size_t)blockIdx.x * NT + tid; i < 5 * KNp; i += (size_t)gridDim.x * NT) a.u_out[i] = a.u_in[i];
+    return;
+  }
+  const double dt = MODE == 0 ? a.scal->dt : 0.0;
+  const double gamma = a.gamma;
+
+  for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
+  {
+    const double *pv = (const double *)(tab + TB::pv_off), *vv = (const double *)(tab + TB::vv_off);
+    const double *pf = (const double *)(tab + TB::pf_off), *vf = (const double *)(tab + TB::vf_off);
+    for (int i = tid; i < Np * 4; i += NT) {
+      s_tabv[0][i] = pv[i];
+      s_tabv[1][i] = vv[i];
+    }
+    for (int i = tid; i < NFF * 4; i += NT) {
+      s_tabf[0][i] = pf[i];
+      s_tabf[1][i] = vf[i];
+    }
+  }
+  for (int i = 3 * Np * EB + tid; i < KV4 * EB; i += NT) s_vr[0][i] = s_vr[1][i] = 0.0f;  // padded rows stay zero
+  const double *pv = s_tabv[0], *vv = s_tabv[1], *pf = s_tabf[0], *vf = s_tabf[1];
+
+  const int grp = blockIdx.x % ngrp, gsl = blockIdx.x / ngrp, gsz = gridDim.x / ngrp;
+  const int lo = (int)((long)nbatch * grp / ngrp), hi = (int)((long)nbatch * (grp + 1) / ngrp);
+
+  // Stages 1-3 of a batch, run by every wave (each role calls it at the same
+  // point of its own loop; both roles pass the same barriers).
+  auto stage_in = [&](int e0, int ne) {
+    mx_barrier();  // the previous batch is done with every LDS buffer
+    // ---- 1. node values (columns past the last element: a state at rest),
+    //      geometry, connectivity, tau; the loads issued together ----
+    {
+      double ld[5][NLD];
+#pragma unroll
+      for (int v = 0; v < 5; v++)
+#pragma unroll
+        for (int x = 0; x < NLD; x++) {
+          const int i = tid + NT * x;
+          ld[v][x] = i < ne * Np ? a.u_in[v * KNp + (size_t)e0 * Np + i] : (v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0));
+        }
+#pragma unroll
+      for (int x = 0; x < NLD; x++) {
+        const int i = tid + NT * x, e = i / Np, j = i - e * Np;
+        if (i < EB * Np)
+#pragma unroll
+          for (int v = 0; v < 5; v++) s_u[v][e * UST + j] = ld[v][x];
+      }
+    }
+    for (int i = tid; i < EB * GEO_STRIDE; i += NT)
+      s_geo[i] = i < ne * GEO_STRIDE ? a.geo[(size_t)e0 * GEO_STRIDE + i] : (i % GEO_STRIDE < 9 ? 0.0 : 1.0);
+    if (tid < EB * 4) {
+      const bool live = tid < ne * 4;
+      const int c = live ? a.conn[(size_t)e0 * 4 + tid] : 0;
+      s_conn[tid] = c;
+      s_tau[tid] = live ? 0.25 * (a.maxv[(size_t)e0 * 4 + tid] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) * a.tau_scale
+                        : 0.0;
+    }
+    mx_barrier();
+    // ---- 2. surface fluxes of all five variables at every face-node pair,
+    //      their least-squares face fits and the FP32 remainders.  Thread
+    //      (face fi = (f, e), group s) takes face nodes jf = s + 8x; the
+    //      fit sums meet across the eight groups of a face in registers ----
+    {
+      const int fi = tid >> 3, s8 = tid & 7, e = fi & 15, f = fi >> 4;
+      const int cc = s_conn[e * 4 + f];
+      const double *g = &s_geo[e * GEO_STRIDE + 9 + f * 4];
+      const double tau = s_tau[e * 4 + f];
+      double fl[JF][5];
+      {
+        double uP[JF][5];  // exterior traces, gathered together
+#pragma unroll
+        for (int x = 0; x < JF; x++) {
+          const int jf = s8 + 8 * x;
+          size_t pidx = 0;  // dead pairs read node 0 (unused)
+          if (jf < Nfp && e < ne)
+            pidx = (size_t)conn_nbr(cc) * Np + s_fmx[(conn_face(cc) * 4 + conn_code(cc)) * Nfp + jf];
+#pragma unroll
+          for (int v = 0; v < 5; v++) uP[x][v] = a.u_in[v * KNp + pidx];
+        }
+#pragma unroll
+        for (int x = 0; x < JF; x++) {
+          const int jf = s8 + 8 * x;
+#pragma unroll
+          for (int v = 0; v < 5; v++) fl[x][v] = 0.0;
+          if (jf < Nfp && e < ne) {
+            const int q = e * UST + s_fmx[(f * 4 + 3) * Nfp + jf];
+            double uM[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) uM[v] = s_u[v][q];
+            mx_surface_flux5(uM, uP[x], g, tau, a.flux, gamma, fl[x]);
+          }
+        }
+      }
+      double c[5][3] = {};
+#pragma unroll
+      for (int x = 0; x < JF; x++) {
+        const int jf = s8 + 8 * x;
+        if (jf < Nfp) {
+          const double *P = &pf[(f * Nfp + jf) * 4];
+#pragma unroll
+          for (int v = 0; v < 5; v++)
+#pragma unroll
+            for (int qq = 0; qq < 3; qq++) c[v][qq] += P[qq] * fl[x][v];
+        }
+      }
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+          for (int qq = 0; qq < 3; qq++) c[v][qq] += __shfl_xor(c[v][qq], o, 64);
+      if (s8 == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+          for (int qq = 0; qq < 3; qq++) s_c[v][(12 + f * 3 + qq) * EB + e] = c[v][qq];
+#pragma unroll
+      for (int x = 0; x < JF; x++) {
+        const int jf = s8 + 8 * x;
+        if (jf < Nfp) {
+          const int m = f * Nfp + jf;
+          const double *V = &vf[m * 4];
+#pragma unroll
+          for (int v = 0; v < 5; v++)
+            s_sr[v][m * EB + e] =
+                e < ne ? (float)(fl[x][v] - (V[0] * c[v][0] + V[1] * c[v][1] + V[2] * c[v][2])) : 0.0f;
+        }
+      }
+    }
+    // ---- 3. variable 0's volume pass ----
+    mx_volume_pass<N>(0, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[0], s_vr[0]);
+    mx_barrier();
+  };
+
+  if (!cw) {
+    // ---- helper waves: their share of the stages and of every volume pass ----
+    for (int bq = lo + gsl; bq < hi; bq += gsz) {
+      const int b = order ? order[bq] : bq;
+      const int e0 = b * EB, ne = min(EB, K - e0);
+      stage_in(e0, ne);
+#pragma unroll 1
+      for (int v = 0; v < 5; v++) {
+        if (v < 4)
+          mx_volume_pass<N>(v + 1, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
+        mx_barrier();
+      }
+    }
+    return;
+  }
+
+  // ---- contraction waves: this wave's operator rows (all k-steps) and (A V)
+  //      rows, resident for the whole kernel ----
+  float afr[KS];
+  double avr[NC / 4];
+#pragma unroll
+  for (int x = 0; x < KS; x++) afr[x] = afrag[((size_t)w * KS + x) * 64 + lane];
+#pragma unroll
+  for (int x = 0; x < NC / 4; x++) avr[x] = avfrag[((size_t)w * (NC / 4) + x) * 64 + lane];
+  const int kq = lane >> 4, ec = lane & 15;  // MFMA fragment row group and column
+
+  for (int bq = lo + gsl; bq < hi; bq += gsz) {
+    const int b = order ? order[bq] : bq;
+    const int e0 = b * EB, ne = min(EB, K - e0);
+    stage_in(e0, ne);
+    // ---- 4. per variable: the contraction, the fit term, the update, then
+    //      this wave's share of the next variable's volume pass ----
+#pragma unroll 1
+    for (int v = 0; v < 5; v++) {
+      // the LSRK register of rows 16w + 4kq + r, loaded ahead of the chain
+      double kold[4] = {0.0, 0.0, 0.0, 0.0};
+      if (MODE == 0 && ec < ne)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * w + 4 * kq + r;
+          if (i < Np) kold[r] = a.k[v * KNp + (size_t)(e0 + ec) * Np + i];
+        }
+      const float *vr = s_vr[v & 1];
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      // (a scheduling fence every 8 k-steps bounds the LDS reads hoisted
+      // ahead of the chain, and with them the live registers)
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const float bf = ks < KSV ? vr[(4 * ks + kq) * EB + ec] : s_sr[v][(4 * (ks - KSV) + kq) * EB + ec];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks], bf, acc, 0, 0, 0);
+        if (ks % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      f64x4 acc64 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < NC / 4; ks++)
+        acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(avr[ks], s_c[v][(4 * ks + kq) * EB + ec], acc64, 0, 0, 0);
+      if (ec < ne) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * w + 4 * kq + r;
+          if (i < Np) {
+            const double rhs = (double)acc[r] + acc64[r];
+            const size_t gi = v * KNp + (size_t)(e0 + ec) * Np + i;
+            if (MODE == 1) {
+              a.rhs[gi] = rhs;
+            } else {
+              const double kn = a.rka * kold[r] + dt * rhs;
+              a.k[gi] = kn;
+              a.u_out[gi] = s_u[v][ec * UST + i] + a.rkb * kn;
+            }
+          }
+        }
+      }
+      if (v < 4)
+        mx_volume_pass<N>(v + 1, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
+      mx_barrier();  // remainder tile of v + 1 complete; the tile of v is free
+    }
+  }
+}
+
+}  // namespace dglm

@@ -27,6 +27,7 @@
 #include "kernels/dg_step.hpp"
 #include "kernels/dg_mfma.hpp"
 #include "kernels/dg_ws.hpp"
+#include "kernels/dg_mx.hpp"
 #include "kernels/dg_adjoint.hpp"
 #include "kernels/dg_adjoint_mfma.hpp"
 
@@ -90,6 +91,8 @@ struct dglm_ctx {
   double *afrag_ws = nullptr;     // operator rows in the warp-specialised kernel's k order
   double *arem_ws = nullptr;      // rows left to the VALU by that kernel
   int *ws_order = nullptr;        // its batch visiting order (null: mesh order)
+  unsigned char *mx_tab = nullptr;  // tables of the mixed-precision Euler kernel (null: not built)
+  int mx_grid = 0, mx_groups = 1;    // its persistent grid and XCD groups
   int ws_grid = 0, ws_groups = 1;  // its persistent grid and XCD groups
   struct dglm_adj_tape *tape = nullptr;  // adjoint buffers and recorded trajectory
   bool use_graph = true;                 // replay two-step hipGraphs in dglm_step
@@ -288,6 +291,7 @@ int stage_kind(const dglm_ctx *c) {
   const bool ws = c->afrag_ws != nullptr, mf = mfma_available(c->p.N) && c->afrag != nullptr;
   if (c->variant == DGLM_STAGE_SIMPLE && c->p.precision == DGLM_FP64) return DGLM_STAGE_SIMPLE;
   if (c->variant == DGLM_STAGE_MFMA && mf) return DGLM_STAGE_MFMA;
+  if (c->mx_tab) return DGLM_STAGE_MFMA_MX;
   if (ws) return DGLM_STAGE_MFMA_WS;
   return mf ? DGLM_STAGE_MFMA : DGLM_STAGE_SIMPLE;
 }
@@ -320,6 +324,29 @@ void launch_ws(dglm_ctx *c, const StageArgs &a) {
   }
 }
 
+template <int N, int MODE>
+void launch_mx(dglm_ctx *c, const StageArgs &a) {
+  if constexpr (N <= 6) {
+    using S = MxShape<N>;
+    const int nbatch = (c->K + S::EB - 1) / S::EB;
+    if (c->mx_grid == 0) {
+      int dev = 0, cus = 0, occ = 0;
+      hipGetDevice(&dev);
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, stage_mx_kernel<N, 0>, S::NT, 0);
+      c->mx_grid = (cus > 0 ? cus : 256) * (occ > 0 ? occ : 1);
+      c->mx_groups = c->mx_grid % 8 == 0 ? 8 : 1;
+    }
+    int grid = c->mx_grid, groups = c->mx_groups;
+    if (nbatch < grid) {
+      grid = nbatch;
+      groups = 1;
+    }
+    hipLaunchKernelGGL((stage_mx_kernel<N, MODE>), dim3(grid), dim3(S::NT), 0, c->stream, a, c->mx_tab, c->ws_order,
+                       nbatch, groups);
+  }
+}
+
 template <int N, int PHYS, int MODE, int PREC>
 void launch_mfma(dglm_ctx *c, const StageArgs &a) {
   using S = MfmaShape<N>;
@@ -341,6 +368,10 @@ void launch_stage(dglm_ctx *c, const StageArgs &a) {
   const int kind = stage_kind(c);
   if (PHYS == PHYS_EULER && kind == DGLM_STAGE_MFMA_WS) {
     launch_ws<N, MODE>(c, a);
+    return;
+  }
+  if (PHYS == PHYS_EULER && kind == DGLM_STAGE_MFMA_MX) {
+    launch_mx<N, MODE>(c, a);
     return;
   }
   if constexpr (N <= 6) {
@@ -440,7 +471,7 @@ void free_ctx(dglm_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   void *ptrs[] = {c->u[0], c->u[1], c->k, c->rhs, c->vel, c->geo, c->op, c->maxv, c->partial, c->dt_hist,
                   c->conn, c->fmx, c->scal, c->afrag, c->rowsum, c->partial_idx, c->afrag_ws, c->arem_ws,
-                  c->ws_order, c->dmap, c->gbnd};
+                  c->ws_order, c->dmap, c->gbnd, c->mx_tab};
   for (void *q : ptrs)
     if (q) hipFree(q);
   free_tape(c->tape);
@@ -644,12 +675,36 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
     }
   }
 
+  // Tables of the fused mixed-precision Euler kernel (no Dirichlet faces:
+  // those take the generic MFMA kernel).
+  std::vector<unsigned char> mx_tab;
+  if (p->physics == DGLM_EULER && p->precision == DGLM_FP32 && dface.empty() && !rsum.empty() &&
+      (long long)K * Np < (1LL << 31)) {
+    const double *fitV = rsum.data(), *fitP = rsum.data() + 4 * (size_t)KD, *fitAV = rsum.data() + 8 * (size_t)KD;
+    switch (p->N) {
+#define DGLM_MX_CASE(n)                                                  \
+  case n:                                                              \
+    mx_tab.assign(MxTables<n>::bytes, 0);                              \
+    mx_build_tables<n>(op.data(), fitV, fitP, fitAV, mx_tab.data());   \
+    break;
+      DGLM_MX_CASE(1)
+      DGLM_MX_CASE(2)
+      DGLM_MX_CASE(3)
+      DGLM_MX_CASE(4)
+      DGLM_MX_CASE(5)
+      DGLM_MX_CASE(6)
+#undef DGLM_MX_CASE
+      default:
+        break;
+    }
+  }
+
   // Batch order of the warp-specialised kernel: 16-element batches sorted by
   // the Morton code of their node centroid, so that the workgroups of one XCD,
   // which walk a contiguous stretch of the order together, work on a compact
   // region and find the neighbours' face traces in their L2.
   std::vector<int> ws_order;
-  if (!afr_ws.empty() && m->px && m->py && m->pz) {
+  if ((!afr_ws.empty() || !mx_tab.empty()) && m->px && m->py && m->pz) {
     const int nb = (K + 15) / 16;
     const size_t KNp = (size_t)K * Np;
     double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
@@ -716,7 +771,8 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       (!afr_ws.empty() && (rc = dev_alloc(&c->afrag_ws, afr_ws.size(), "A-fragments (warp-specialised)"))) ||
       (!arem_ws.empty() && (rc = dev_alloc(&c->arem_ws, arem_ws.size(), "remainder rows"))) ||
       (!ws_order.empty() && (rc = dev_alloc(&c->ws_order, ws_order.size(), "batch order"))) ||
-      (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "affine-offset tables")))) {
+      (!rsum.empty() && (rc = dev_alloc(&c->rowsum, rsum.size(), "affine-offset tables"))) ||
+      (!mx_tab.empty() && (rc = dev_alloc(&c->mx_tab, mx_tab.size(), "mixed-precision tables")))) {
     free_ctx(c);
     return rc;
   }
@@ -751,6 +807,7 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       c->rowsum ? hipMemcpy(c->rowsum, rsum.data(), rsum.size() * sizeof(double), hipMemcpyHostToDevice)
                 : hipSuccess,
       c->dmap ? hipMemcpy(c->dmap, dmap.data(), dmap.size() * sizeof(int), hipMemcpyHostToDevice) : hipSuccess,
+      c->mx_tab ? hipMemcpy(c->mx_tab, mx_tab.data(), mx_tab.size(), hipMemcpyHostToDevice) : hipSuccess,
       hipMemcpy(c->scal, &s0, sizeof(s0), hipMemcpyHostToDevice),
       hipMemset(c->u[0], 0, nst * sizeof(double)),
       hipMemset(c->u[1], 0, nst * sizeof(double)),
@@ -1092,8 +1149,11 @@ extern "C" int dglm_set_graphs(dglm_ctx *c, int on) {
 extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
   if (!c) return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: null context");
   if (variant != DGLM_STAGE_AUTO && variant != DGLM_STAGE_SIMPLE && variant != DGLM_STAGE_MFMA &&
-      variant != DGLM_STAGE_MFMA_WS)
+      variant != DGLM_STAGE_MFMA_WS && variant != DGLM_STAGE_MFMA_MX)
     return fail(DGLM_ERR_ARG, "dglm_set_stage_kernel: unknown variant");
+  if (variant == DGLM_STAGE_MFMA_MX && !c->mx_tab)
+    return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: the fused mixed-precision kernel is Euler with "
+                                      "precision DGLM_FP32, N <= 6 and no Dirichlet faces");
   if (variant == DGLM_STAGE_MFMA_WS && !c->afrag_ws)
     return fail(DGLM_ERR_UNSUPPORTED,
                 "dglm_set_stage_kernel: the warp-specialised kernel is FP64 Euler with N = 3 or 4");

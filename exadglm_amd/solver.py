@@ -167,13 +167,14 @@ class Solver:
         operator rows in registers) or 'mfma_ws' (persistent warp-specialised
         FP64 Euler kernel, N = 3, 4)."""
         code = {"auto": A.DGLM_STAGE_AUTO, "simple": A.DGLM_STAGE_SIMPLE, "mfma": A.DGLM_STAGE_MFMA,
-                "mfma_ws": A.DGLM_STAGE_MFMA_WS}[variant]
+                "mfma_ws": A.DGLM_STAGE_MFMA_WS, "mfma_mx": A.DGLM_STAGE_MFMA_MX}[variant]
         A.check(A.lib().dglm_set_stage_kernel(self._ctx, code))
 
     def stage_kernel(self):
         v = C.c_int()
         A.check(A.lib().dglm_get_stage_kernel(self._ctx, C.byref(v)))
-        return {A.DGLM_STAGE_SIMPLE: "simple", A.DGLM_STAGE_MFMA: "mfma", A.DGLM_STAGE_MFMA_WS: "mfma_ws"}[v.value]
+        return {A.DGLM_STAGE_SIMPLE: "simple", A.DGLM_STAGE_MFMA: "mfma", A.DGLM_STAGE_MFMA_WS: "mfma_ws",
+                A.DGLM_STAGE_MFMA_MX: "mfma_mx"}[v.value]
 
     def set_timing(self, on=True):
         A.check(A.lib().dglm_set_timing(self._ctx, 1 if on else 0))

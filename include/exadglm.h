@@ -235,7 +235,7 @@ int dglm_set_timing(dglm_ctx *ctx, int on);
  * run the MFMA contraction; N = 3, 4).  AUTO takes MFMA_WS where it exists,
  * else MFMA, else SIMPLE (N = 7..10).  dglm_get_stage_kernel reports the
  * kernel that runs. */
-enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2, DGLM_STAGE_MFMA_WS = 3 };
+enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2, DGLM_STAGE_MFMA_WS = 3, DGLM_STAGE_MFMA_MX = 4 };
 int dglm_set_stage_kernel(dglm_ctx *ctx, int variant);
 int dglm_get_stage_kernel(dglm_ctx *ctx, int *variant);
 int dglm_get_timing(dglm_ctx *ctx, double *ms, long *count, int n);

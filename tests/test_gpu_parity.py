@@ -410,17 +410,27 @@ def test_launcher_euler_notebook_flow(gpu, tmp_path, dtype):
     assert e < 3e-7
 
 
-@pytest.mark.parametrize("N", [2, 4, 6])
-def test_mixed_precision_rhs(gpu, N):
-    """FP32-MFMA contraction of the offset-split fluxes: one RHS within FP32
-    rounding of the in-block flux variation (bound 1e-7 of the term scale)."""
-    _say(f"mixed precision rhs N={N}")
+MIXED = ["mfma_mx", "mfma"]  # the fused mixed-precision kernel (AUTO) and the generic one
+
+
+@pytest.mark.parametrize("kernel", MIXED)
+@pytest.mark.parametrize("flux", ["lm", "rusanov"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_mixed_precision_rhs(gpu, N, flux, kernel):
+    """FP32-MFMA contraction of the split fluxes (vertex-interpolation
+    remainders in the fused kernel, least-squares affine remainders in the
+    generic one): one RHS within FP32 rounding of the in-block flux variation
+    (bound 1e-7 of the term scale), both fluxes."""
+    _say(f"mixed precision rhs N={N} {flux} {kernel}")
     pb = cube_problem(N, 3)
     u = smooth_state(pb, "euler", seed=N + 31)
-    with Solver(pb, "euler", precision="fp32") as s:
-        assert s.stage_kernel() == "mfma"
+    with Solver(pb, "euler", precision="fp32", flux=flux) as s:
+        if kernel == "mfma_mx":
+            assert s.stage_kernel() == "mfma_mx"  # AUTO
+        s.set_stage_kernel(kernel)
+        assert s.stage_kernel() == kernel
         r, mv, tau = s.rhs(u)
-    ro, mvo, tauo = O.Oracle(pb, "euler").rhs(u)
+    ro, mvo, tauo = O.Oracle(pb, "euler", flux=flux).rhs(u)
     assert rel_err(mv, mvo) < 1e-15  # wave speed stays FP64
     a = pb.arrays
     opn = max(np.abs(a[k]).sum(axis=1).max() for k in ("WDr", "WDs", "WDt", "LIFT"))
@@ -429,17 +439,19 @@ def test_mixed_precision_rhs(gpu, N):
         assert np.abs(r[v] - ro[v]).max() < 1e-7 * 3 * opn * met * np.abs(u).max() ** 2
 
 
-@pytest.mark.parametrize("N", [4, 6])
-def test_mixed_precision_vortex(gpu, N):
+@pytest.mark.parametrize("kernel", MIXED)
+@pytest.mark.parametrize("N", [4, 5, 6])
+def test_mixed_precision_vortex(gpu, N, kernel):
     """Isentropic vortex, 6^3 hexes, 40 steps: FP32-MFMA contraction with FP64
     state / pressure / wave speed stays within 1e-8 relative L2 of the FP64
     oracle for rho, rho*u, rho*v and E; rho*w (zero in the exact solution) is
     measured against the momentum norm."""
-    _say(f"mixed precision vortex N={N}")
+    _say(f"mixed precision vortex N={N} {kernel}")
     pb = uniform_problem(N, 7, 7, 7, box=vortex_box(6))
     a = pb.arrays
     u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
     with Solver(pb, "euler", cfl=0.4, precision="fp32") as s:
+        s.set_stage_kernel(kernel)
         s.set_state(u0)
         s.step(40)
         u = s.get_state()
@@ -466,6 +478,7 @@ def test_c3_mixed_precision_full_size(gpu, capsys):
     out = {}
     for prec in ("fp64", "fp32"):
         with Solver(pb, "euler", cfl=0.4, flux="rusanov", precision=prec) as s:
+            assert s.stage_kernel() == ("mfma_mx" if prec == "fp32" else "mfma")
             s.set_state(u0)
             if prec == "fp64":
                 s.step(5)
