@@ -33,12 +33,13 @@ struct AdjShape {
   static constexpr int NpA = 4 * KSA;        // padded node count of the rbar tile
   static constexpr int NT = 64 * KT;
   static constexpr int EB = 16;
+  static constexpr int EBP = EB + 1;  // LDS row stride of the rbar and g tiles
 };
 
 template <int N>
 constexpr size_t adj_mfma_lds_bytes() {
   using S = AdjShape<N>;
-  return 8 * (5 * S::NpA * S::EB + 5 * S::KD * S::EB + S::EB * GEO_STRIDE + S::EB * 4 +
+  return 8 * (5 * S::NpA * S::EBP + 5 * S::KD * S::EBP + S::EB * GEO_STRIDE + S::EB * 4 +
               S::NT / 64) +
          4 * (S::EB * 4 + 16 * S::Nfp + S::Np * 3);
 }
@@ -67,10 +68,12 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   using S = AdjShape<N>;
   using P = PrecTraits<PREC_F64>;
   constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, KD = S::KD, KSA = S::KSA, NpA = S::NpA, NT = S::NT;
-  constexpr int EB = S::EB;
+  constexpr int EB = S::EB, EBP = S::EBP;
   static_assert(EB * Np <= NT, "one volume node per thread");
-  __shared__ double s_r[5][NpA * EB];  // rbar, [node][element]
-  __shared__ double s_g[5][KD * EB];   // A^T rbar, [row][element]
+  // rows padded to EBP elements: the phase-2 reads (one node per lane, the
+  // lanes of a wave on consecutive nodes) then fall in distinct banks
+  __shared__ double s_r[5][NpA * EBP];  // rbar, [node][element]
+  __shared__ double s_g[5][KD * EBP];   // A^T rbar, [row][element]
   __shared__ double s_geo[EB * GEO_STRIDE];
   __shared__ double s_tau[EB * 4];
   __shared__ int s_conn[EB * 4];
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   __shared__ double red[NT / 64];
   // tau-bar partial of each face-node slot, [element][slot]; rbar is dead
   // once the contraction is done
-  static_assert(EB * NFF <= 5 * NpA * EB, "slot partials must fit the rbar tile");
+  static_assert(EB * NFF <= 5 * NpA * EBP, "slot partials must fit the rbar tile");
   double *const s_tc = &s_r[0][0];
 
   const int K = a.K;
@@ -94,6 +97,12 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   const bool live = tid < ne * Np;
   const int e = tid / Np, j = tid - e * Np;
   const size_t g0 = (size_t)e0 * Np + tid;
+
+  // A^T fragments of the wave's row tile, first used in phase 1: issued
+  // first so that their latency is hidden behind phase 0
+  double afr[KSA];
+#pragma unroll
+  for (int ks = 0; ks < KSA; ks++) afr[ks] = afragT[((size_t)w * KSA + ks) * 64 + lane];
 
   // ---- phase 0: staging, rbar, kbar update ----
   for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
     lam_nbr = a.maxv[conn_nbr(c) * 4 + conn_face(c)];
   }
   for (int v = 0; v < 5; v++) {
-    for (int i = tid; i < NpA * EB; i += NT) s_r[v][i] = 0.0;  // padded rows and columns
+    for (int i = tid; i < NpA * EBP; i += NT) s_r[v][i] = 0.0;  // padded rows and columns
   }
   const double dt = a.mode == 0 ? *a.dtp : 0.0;
   double U[5], UB[5], R[5];
@@ -141,25 +150,22 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   __syncthreads();  // s_r zeroed
   if (live) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) s_r[v][j * EB + e] = R[v];
+    for (int v = 0; v < 5; v++) s_r[v][j * EBP + e] = R[v];
   }
   __syncthreads();
 
   // ---- phase 1: g = A^T rbar on MFMA, wave w = row tile w ----
   {
-    double afr[KSA];
-#pragma unroll
-    for (int ks = 0; ks < KSA; ks++) afr[ks] = afragT[((size_t)w * KSA + ks) * 64 + lane];
     const int kk = lane >> 4, ec = lane & 15;
 #pragma unroll 1
     for (int v = 0; v < 5; v++) {
       typename P::Acc acc = {0, 0, 0, 0};
 #pragma unroll
-      for (int ks = 0; ks < KSA; ks++) acc = P::mfma(afr[ks], s_r[v][(4 * ks + kk) * EB + ec], acc);
+      for (int ks = 0; ks < KSA; ks++) acc = P::mfma(afr[ks], s_r[v][(4 * ks + kk) * EBP + ec], acc);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int k = 16 * w + 4 * r + kk;
-        if (k < KD) s_g[v][k * EB + ec] = acc[r];
+        if (k < KD) s_g[v][k * EBP + ec] = acc[r];
       }
     }
   }
@@ -197,8 +203,8 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
       double h[5];
 #pragma unroll
       for (int v = 0; v < 5; v++)
-        h[v] = m[0 + x] * s_g[v][j * EB + e] + m[3 + x] * s_g[v][(Np + j) * EB + e] +
-               m[6 + x] * s_g[v][(2 * Np + j) * EB + e];
+        h[v] = m[0 + x] * s_g[v][j * EBP + e] + m[3 + x] * s_g[v][(Np + j) * EBP + e] +
+               m[6 + x] * s_g[v][(2 * Np + j) * EBP + e];
       euler_jt(U, x == 0 ? 1.0 : 0.0, x == 1 ? 1.0 : 0.0, x == 2 ? 1.0 : 0.0, h, a.gamma, wv);
       __builtin_amdgcn_sched_barrier(0);  // one direction at a time: bounds live registers
     }
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
       double tc = 0.0;
 #pragma unroll
       for (int v = 0; v < 5; v++) {
-        const double gm = s_g[v][(3 * Np + mslot) * EB + e];
+        const double gm = s_g[v][(3 * Np + mslot) * EBP + e];
         ub[v] = 0.5 * (U[v] + uP[q][v]);
         h[v] = 0.5 * g[3] * gm;
         tc += gm * (-(ub[v] - U[v]) * g[3]);
