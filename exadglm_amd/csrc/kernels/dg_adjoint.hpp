@@ -61,6 +61,9 @@ struct AdjArgs {
   double *kbar;        // [NV][K*Np]  in/out (mode 0)
   const double *rbar;  // [NV][K*Np]  (mode 1)
   double *gsur;        // [K][NFF][NV] partner-side face terms (pass A out, pass B in)
+  // partner-side face terms of the previous pass A, not yet added to ubar by
+  // a pass B: the MFMA pass A adds them to ubar itself (null: none pending)
+  const double *gsur_in;
   double *taubar;      // [K*4] accumulated tau adjoint per face side (may be null)
   const double *vel;
   const double *geo;

@@ -9,7 +9,9 @@
 //     the A^T fragments of its rows for the k-steps over the Np nodes in
 //     registers and runs ceil(Np/4) v_mfma_f64_16x16x4f64 per variable;
 //   * each thread owns one volume node of the 16 elements; phase 0 loads its
-//     u, ubar and kbar into registers, updates kbar and puts
+//     u, ubar and kbar into registers, adds to ubar the partner-side face
+//     terms the previous stage's pass A left for the node (when given: the
+//     work of the pass B of that stage), updates kbar and puts
 //     rbar = dt (kbar + b ubar) into an LDS tile [node][element];
 //   * phase 1 is the contraction, g[v][k][e] into LDS (all five variables,
 //     since the transposed flux Jacobians mix them);
@@ -41,7 +43,7 @@ constexpr size_t adj_mfma_lds_bytes() {
   using S = AdjShape<N>;
   return 8 * (5 * S::NpA * S::EBP + 5 * S::KD * S::EBP + S::EB * GEO_STRIDE + S::EB * 4 +
               S::NT / 64) +
-         4 * (S::EB * 4 + 16 * S::Nfp + S::Np * 3);
+         4 * (S::EB * 4 + 16 * S::Nfp + S::Np * 3 + 4 * S::Np);
 }
 
 template <int N>
@@ -79,6 +81,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   __shared__ int s_conn[EB * 4];
   __shared__ int s_fmx[16 * Nfp];
   __shared__ int s_slots[Np * 3];
+  __shared__ int s_fpos[4 * Np];
   __shared__ double red[NT / 64];
   // tau-bar partial of each face-node slot, [element][slot]; rbar is dead
   // once the contraction is done
@@ -107,6 +110,8 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   // ---- phase 0: staging, rbar, kbar update ----
   for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
   for (int i = tid; i < Np * 3; i += NT) s_slots[i] = a.node_slots[i];
+  if (a.gsur_in)
+    for (int i = tid; i < 4 * Np; i += NT) s_fpos[i] = a.fpos[i];
   for (int i = tid; i < ne * GEO_STRIDE; i += NT) s_geo[i] = a.geo[(size_t)e0 * GEO_STRIDE + i];
   // wave speeds for tau: loaded here, written to LDS after the contraction
   // (tau is first read in phase 2), so their dependent loads do not hold up
@@ -124,6 +129,7 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
   }
   const double dt = a.mode == 0 ? *a.dtp : 0.0;
   double U[5], UB[5], R[5];
+  int cq[3] = {0, 0, 0};  // connectivity words of the node's faces (pending partner terms)
   if (live) {
     // every load before any store (kbar may alias nothing, but the compiler
     // cannot know): one round trip instead of five
@@ -138,16 +144,52 @@ __global__ __launch_bounds__(AdjShape<N>::NT) void adjoint_pass_a_mfma(AdjArgs a
         R[v] = a.rbar[v * KNp + g0];
       }
     }
-    if (a.mode == 0) {
+    if (a.gsur_in) {
 #pragma unroll
-      for (int v = 0; v < 5; v++) {
-        const double kb = R[v] + a.b * UB[v];
-        a.kbar[v * KNp + g0] = a.a * kb;
-        R[v] = dt * kb;
+      for (int q = 0; q < 3; q++) {
+        const int ms = a.node_slots[j * 3 + q];
+        if (ms >= 0) cq[q] = a.conn[(size_t)(e0 + e) * 4 + ms / Nfp];
       }
     }
   }
-  __syncthreads();  // s_r zeroed
+  __syncthreads();  // s_r zeroed; s_fmx, s_slots, s_fpos
+  if (live && a.gsur_in) {
+    // the previous stage's pass-B work: ubar += the partner-side terms the
+    // neighbours' face-node slots stored for this node (same sum as
+    // adjoint_pass_b, so ubar is bitwise what that pass would leave)
+    double t[3][5];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int ms = s_slots[j * 3 + q];
+#pragma unroll
+      for (int v = 0; v < 5; v++) t[q][v] = 0.0;
+      if (ms >= 0) {
+        const int f = ms / Nfp, jj = ms - f * Nfp;
+        const int c = cq[q];
+        const int nb = conn_nbr(c), nf = conn_face(c);
+        const int pnode = s_fmx[(nf * 4 + conn_code(c)) * Nfp + jj];  // neighbour's volume node
+        const int jp = s_fpos[nf * Np + pnode];                       // its slot on face nf
+        const double *gs = &a.gsur_in[((size_t)nb * NFF + nf * Nfp + jp) * 5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) t[q][v] = gs[v];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      double wsum = 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; q++) wsum += t[q][v];
+      UB[v] += wsum;
+    }
+  }
+  if (live && a.mode == 0) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const double kb = R[v] + a.b * UB[v];
+      a.kbar[v * KNp + g0] = a.a * kb;
+      R[v] = dt * kb;
+    }
+  }
   if (live) {
 #pragma unroll
     for (int v = 0; v < 5; v++) s_r[v][j * EBP + e] = R[v];

@@ -1349,6 +1349,10 @@ struct dglm_adj_tape {
   int *fstar = nullptr, *clip = nullptr, *ck_step = nullptr;
   DevScalars *scal2 = nullptr;
   double *ubar = nullptr, *kbar = nullptr, *gsur = nullptr, *taubar = nullptr, *dtpart = nullptr, *tsc = nullptr;
+  // second face-term buffer of the MFMA pass A, which takes over the pass B
+  // of stages 4..1 (each pass A reads the terms of the one before it while
+  // writing its own)
+  double *gsur2 = nullptr;
   double *rbar = nullptr;
   int *slots = nullptr, *fpos = nullptr;
   // The last nkeep segments kept whole: their stage states [5*every + 1][nst]
@@ -1382,7 +1386,7 @@ void launch_lam(dglm_ctx *c, const double *u, const double *lam, int step) {
 void free_tape(dglm_adj_tape *T) {
   if (!T) return;
   void *ptrs[] = {T->keep, T->keep_lam, T->ckpt, T->ck_t, T->dt, T->seg, T->lam, T->kscr, T->fstar, T->clip, T->ck_step, T->scal2,
-                  T->ubar, T->kbar, T->gsur, T->taubar, T->dtpart, T->tsc, T->rbar, T->slots, T->fpos,
+                  T->ubar, T->kbar, T->gsur, T->gsur2, T->taubar, T->dtpart, T->tsc, T->rbar, T->slots, T->fpos,
                   T->afragT};
   for (void *q : ptrs)
     if (q) hipFree(q);
@@ -1449,6 +1453,9 @@ int ensure_adjoint_buffers(dglm_ctx *c) {
     }
     if ((rc = dev_alloc(&T->afragT, frag.size(), "A^T fragments"))) return rc;
     HIP_TRY(hipMemcpy(T->afragT, frag.data(), frag.size() * sizeof(double), hipMemcpyHostToDevice));
+    const size_t ng = (size_t)c->nvar * c->K * c->NFF;
+    if ((rc = dev_alloc(&T->gsur2, ng, "face adjoint (second)"))) return rc;
+    T->bytes += ng * sizeof(double);
   }
   return DGLM_OK;
 }
@@ -1551,7 +1558,8 @@ extern "C" int dglm_adjoint_record(dglm_ctx *c, int nsteps, int every) {
     T->nsteps = nsteps;
     T->every = every;
     T->nck = nck;
-    T->bytes = (2 * nst + (size_t)c->nvar * c->K * c->NFF + (size_t)c->K * 4 + T->nblk_a) * sizeof(double) +
+    T->bytes = (2 * nst + (size_t)c->nvar * c->K * c->NFF * (T->gsur2 ? 2 : 1) + (size_t)c->K * 4 + T->nblk_a) *
+                   sizeof(double) +
                (nst * (nck + 5 * (size_t)every + 2) + (size_t)c->K * 4 * every) * sizeof(double);
   }
   // Segments kept whole: HBM bought back from the recompute.  The count is the
@@ -1690,6 +1698,13 @@ extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *
       for (int s = 4; s >= 0; s--) {
         AdjArgs a = adj_args(c);
         a.mode = 0;
+        if (T->gsur2) {
+          // the MFMA pass A adds the partner terms of the stage before it
+          // (written to the other buffer); only stage 0's go through pass B,
+          // ahead of the wave-speed adjoint
+          a.gsur = (s & 1) ? T->gsur2 : T->gsur;
+          a.gsur_in = s < 4 ? ((s & 1) ? T->gsur : T->gsur2) : nullptr;
+        }
         a.u = base + nst * s;
         a.maxv = lam;
         a.dtp = T->dt + n;
@@ -1703,7 +1718,7 @@ extern "C" int dglm_adjoint_backward(dglm_ctx *c, const double *ubar_T, double *
           a.dtpart = T->dtpart;
         }
         A(c, a);
-        B(c, a);
+        if (!T->gsur2 || s == 0) B(c, a);
       }
       if (exact) {
         hipLaunchKernelGGL(adjoint_dt_reduce, dim3(1), dim3(256), 0, c->stream, T->dtpart, T->nblk_a, T->clip, n,
