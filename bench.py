@@ -145,6 +145,12 @@ def mixed_precision_leg(order=6, nhex=24, steps=490, warmup=10):
     r64 = run_forward(pb, u0, "fp64", "rusanov", steps, warmup, 0)
     r32 = run_forward(pb, u0, "fp32", "rusanov", steps, warmup, 0)
     exact = isentropic_vortex(a["px"], a["py"], a["pz"], t=r64["t"])
+    split = {}
+    for prec in ("bf16x3", "bf16x6"):
+        rb = run_forward(pb, u0, prec, "rusanov", steps, warmup, 0)
+        split[prec] = {"ms_per_step": rb["wall"] / steps * 1e3, "speedup_vs_fp64": r64["wall"] / rb["wall"],
+                       "kernel": rb["kernel"], "rel_l2_vs_fp64": per_var_err(rb["u"], r64["u"])}
+        del rb
     return {
         "workload": f"euler_isentropic_vortex_N{order}_{nhex}cube_hexes_x6_K{pb.K}_rusanov_{warmup + steps}steps",
         "fp64": {"value": dof_stage / r64["wall"], "ms_per_step": r64["wall"] / steps * 1e3, "kernel": r64["kernel"]},
@@ -156,6 +162,9 @@ def mixed_precision_leg(order=6, nhex=24, steps=490, warmup=10):
         "rel_l2_mixed_vs_fp64": per_var_err(r32["u"], r64["u"]),
         "rel_l2_fp64_vs_analytic": per_var_err(r64["u"], exact),
         "rel_l2_mixed_vs_analytic": per_var_err(r32["u"], exact),
+        "split_bf16": dict(split, contraction="bf16 MFMA on the same affine-fit remainders cut into bf16 pieces: "
+                                              "x3 = hi*hi + hi*lo + lo*hi, x6 = three pieces, six products; FP32 "
+                                              "accumulation, generic per-variable kernel"),
         "unit": "DOF-stage updates/s",
         "note": "rho*w errors relative to the momentum norm; target rel L2 <= 1e-8 at >= 1.8x",
     }

@@ -23,6 +23,7 @@ DGLM_EULER = 1
 DGLM_FP64 = 0
 DGLM_FP32 = 1
 DGLM_BF16X3 = 2
+DGLM_BF16X6 = 3
 DGLM_FLUX_LM = 0
 DGLM_FLUX_RUSANOV = 1
 DGLM_STAGE_AUTO = 0

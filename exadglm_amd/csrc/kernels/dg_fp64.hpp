@@ -115,7 +115,8 @@ struct StageArgs {
   const DevScalars *scal;
   double rka, rkb;
   double gamma, tau_scale;
-  int flux;  // 0: LM (reference), 1: Rusanov (upwind for advection)
+  int flux;      // 0: LM (reference), 1: Rusanov (upwind for advection)
+  int bf_terms;  // split-BF16 contraction: 3 (two pieces per operand) or 6 (three pieces)
   // Dirichlet faces (EtoB == 1): dmap[e*4+f] = slot of the face (-1: not
   // Dirichlet), gbnd[v][slot][j] = prescribed state at its face node j (Fmask
   // order); nd slots.  dmap null: the mesh has none.

@@ -25,6 +25,15 @@
 //             fluxes, pressure, wave speed and fit are FP64; only the
 //             second-order remainder of each block is rounded to FP32.  D
 //             register r of lane l is row 4(l/16) + r, column l%16.
+//   PREC_BF   the same split, with the remainder and the operator each cut
+//             into bf16 pieces x = x1 + x2 + x3 (x1 = bf16(x), x2 =
+//             bf16(x - x1), ...) and contracted on v_mfma_f32_16x16x16_bf16:
+//             a.bf_terms = 3 adds x1 y1 + x1 y2 + x2 y1 (the split-BF16
+//             three-term product), 6 also x1 y3 + x2 y2 + x3 y1.  The leading
+//             product and the cross terms go to separate FP32 accumulators,
+//             summed in FP64 in the epilogue.  Lane l holds A[l%16][4(l/16)+q]
+//             and B[4(l/16)+q][l%16], q = 0..3, of each 16-deep k-step; D as
+//             PREC_F32.
 #pragma once
 #include <cmath>
 #include <utility>
@@ -35,8 +44,29 @@ namespace dglm {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
 
-enum { PREC_F64 = 0, PREC_F32 = 1 };
+enum { PREC_F64 = 0, PREC_F32 = 1, PREC_BF = 2 };
+constexpr int BF_PIECES = 3;  // bf16 pieces kept per operand value (6-term product)
+
+// bf16 bits of x: round to nearest even of the FP32 value of x (finite x).
+__host__ __device__ inline unsigned short bf16_bits(double x) {
+  const float f = (float)x;
+  unsigned u = __builtin_bit_cast(unsigned, f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline double bf16_value(unsigned short h) {
+  return (double)__builtin_bit_cast(float, (unsigned)h << 16);
+}
+// x = sum of BF_PIECES bf16 pieces plus a remainder below the last one's ulp.
+__host__ __device__ inline void bf16_split(double x, unsigned short (&h)[BF_PIECES]) {
+#pragma unroll
+  for (int p = 0; p < BF_PIECES; p++) {
+    h[p] = bf16_bits(x);
+    x -= bf16_value(h[p]);
+  }
+}
 constexpr int NBLK = 7;  // offset blocks: 3 volume directions + 4 faces
 
 template <int N>
@@ -46,6 +76,7 @@ struct MfmaShape {
   static constexpr int KS = (O::KD + 3) / 4;    // k-steps of 4
   static constexpr int KDP = KS * 4;            // padded contraction length
   static constexpr int NT = RT * 64;            // threads per workgroup
+  static constexpr int KS16 = (O::KD + 15) / 16;  // k-steps of 16 (split-BF16 contraction)
 };
 
 template <int PREC>
@@ -68,6 +99,15 @@ struct PrecTraits<PREC_F32> {
   }
   __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
+template <>
+struct PrecTraits<PREC_BF> {
+  typedef float T;
+  typedef f32x4 Acc;
+  __device__ static inline Acc mfma(s16x4 a, s16x4 b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
 
 // Host-side arrangement of the stacked operator into A-fragments:
 // frag[(w*KS + ks)*64 + lane] = A[16w + lane%16][4ks + lane/16] (0 outside).
@@ -78,6 +118,22 @@ inline void arrange_a_fragments(int Np, int KD, int RT, int KS, const double *op
       for (int l = 0; l < 64; l++) {
         int i = 16 * w + (l & 15), k = 4 * ks + (l >> 4);
         frag[((size_t)w * KS + ks) * 64 + l] = (T)((i < Np && k < KD) ? opT[(size_t)k * Np + i] : 0.0);
+      }
+}
+
+// Host-side split-BF16 A-fragments: frag[((w*KS16 + ks)*BF_PIECES + p)*64 + l]
+// holds piece p of A[16w + l%16][16ks + 4(l/16) + q], q = 0..3 (0 outside).
+inline void arrange_a_bf16(int Np, int KD, int RT, int KS16, const double *opT /*[KD][Np]*/, s16x4 *frag) {
+  for (int w = 0; w < RT; w++)
+    for (int ks = 0; ks < KS16; ks++)
+      for (int l = 0; l < 64; l++) {
+        unsigned short h[4][BF_PIECES];
+        for (int q = 0; q < 4; q++) {
+          const int i = 16 * w + (l & 15), k = 16 * ks + 4 * (l >> 4) + q;
+          bf16_split((i < Np && k < KD) ? opT[(size_t)k * Np + i] : 0.0, h[q]);
+        }
+        for (int p = 0; p < BF_PIECES; p++)
+          for (int q = 0; q < 4; q++) frag[(((size_t)w * KS16 + ks) * BF_PIECES + p) * 64 + l][q] = (short)h[q][p];
       }
 }
 
@@ -244,9 +300,14 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
   constexpr int EB = 16;
   constexpr int NVEL = PHYS == PHYS_ADV ? 3 : 1;
   constexpr bool OFFS = PREC != PREC_F64;
+  constexpr bool BF = PREC == PREC_BF;
+  constexpr int KS16 = S::KS16;
 
   __shared__ double s_u[NV][EB * Np];
-  __shared__ T s_f[KDP * EB];
+  __shared__ T s_f[BF ? 1 : KDP * EB];
+  // split-BF16 remainder tile: piece p of rows 16ks + 4q + 0..3 of column e
+  // at s_fb[p][(4ks + q)*EB + e]
+  __shared__ s16x4 s_fb[BF ? BF_PIECES : 1][BF ? 4 * KS16 * EB : 1];
   __shared__ double s_vel[NVEL][PHYS == PHYS_ADV ? EB * Np : 1];
   __shared__ double s_geo[EB * GEO_STRIDE];
   __shared__ double s_tau[EB * 4];
@@ -273,12 +334,26 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
   const double dt = MODE == 0 ? a.scal->dt : 0.0;
 
   // Operator rows of this wave, resident for the whole kernel.
-  T afr[KS];
+  T afr[BF ? 1 : KS];
+  s16x4 afb[BF ? KS16 : 1][BF_PIECES];
+  if constexpr (BF) {
+    const s16x4 *ab = (const s16x4 *)afrag_v;
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) afr[ks] = afrag[((size_t)w * KS + ks) * 64 + lane];
+    for (int ks = 0; ks < KS16; ks++)
+#pragma unroll
+      for (int p = 0; p < BF_PIECES; p++) afb[ks][p] = ab[(((size_t)w * KS16 + ks) * BF_PIECES + p) * 64 + lane];
+  } else {
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) afr[ks] = afrag[((size_t)w * KS + ks) * 64 + lane];
+  }
 
   for (int i = tid; i < 16 * Nfp; i += NT) s_fmx[i] = a.fmx[i];
-  for (int i = KD * EB + tid; i < KDP * EB; i += NT) s_f[i] = (T)0;  // padded rows stay zero
+  if constexpr (BF) {
+    // padded rows stay zero
+    for (int i = tid; i < BF_PIECES * 4 * KS16 * EB; i += NT) s_fb[i / (4 * KS16 * EB)][i % (4 * KS16 * EB)] = s16x4{0, 0, 0, 0};
+  } else {
+    for (int i = KD * EB + tid; i < KDP * EB; i += NT) s_f[i] = (T)0;  // padded rows stay zero
+  }
 
   // LM surface flux of variable v at face node m of batch element e.
   auto lm_flux = [&](int v, int e, int m) -> double {
@@ -414,17 +489,44 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
           const double *cb = &s_c[offset_block(k, Np, Nfp) * 4 * EB + e];
           const double *V = &fitV[k * 4];
           const double fv = cb[0] * V[0] + cb[EB] * V[1] + cb[2 * EB] * V[2] + cb[3 * EB] * V[3];
-          s_f[idx] = (T)(s_b64[idx] - fv);
+          if constexpr (BF) {
+            unsigned short h[BF_PIECES];
+            bf16_split(s_b64[idx] - fv, h);
+            const int pos = ((k >> 4) * 4 + ((k >> 2) & 3)) * EB + e;
+#pragma unroll
+            for (int p = 0; p < BF_PIECES; p++) s_fb[p][pos][k & 3] = (short)h[p];
+          } else {
+            s_f[idx] = (T)(s_b64[idx] - fv);
+          }
         }
         __syncthreads();
       }
 
       // ---- contraction on MFMA ----
-      typename P::Acc acc = {0, 0, 0, 0};
+      typename P::Acc acc = {0, 0, 0, 0}, acl = {0, 0, 0, 0};
+      if constexpr (BF) {
+        // cross terms into acl (smallest first), the leading product into acc
+        const bool six = a.bf_terms == 6;
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++) {
-        const T bf = s_f[(ks * 4 + (lane >> 4)) * EB + (lane & 15)];
-        acc = P::mfma(afr[ks], bf, acc);
+        for (int ks = 0; ks < KS16; ks++) {
+          const int pos = (ks * 4 + (lane >> 4)) * EB + (lane & 15);
+          const s16x4 b0 = s_fb[0][pos], b1 = s_fb[1][pos];
+          if (six) {
+            const s16x4 b2 = s_fb[2][pos];
+            acl = P::mfma(afb[ks][2], b0, acl);
+            acl = P::mfma(afb[ks][1], b1, acl);
+            acl = P::mfma(afb[ks][0], b2, acl);
+          }
+          acl = P::mfma(afb[ks][1], b0, acl);
+          acl = P::mfma(afb[ks][0], b1, acl);
+          acc = P::mfma(afb[ks][0], b0, acc);
+        }
+      } else {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+          const T bf = s_f[(ks * 4 + (lane >> 4)) * EB + (lane & 15)];
+          acc = P::mfma(afr[ks], bf, acc);
+        }
       }
 
       // ---- LSRK epilogue (FP64) for element lane%16 ----
@@ -435,6 +537,7 @@ __global__ __launch_bounds__(MfmaShape<N>::NT) void stage_mfma_kernel(StageArgs 
           const int i = 16 * w + P::row(lane, r);
           if (i < Np) {
             double rhs = (double)acc[r];
+            if (BF) rhs += (double)acl[r];
             if (OFFS) {
               // exact FP64 contraction of the fit: sum_b sum_q c_bq (A_b V_b)_q
               const double *av = &fitAV[(size_t)i * NBLK * 4];

@@ -378,6 +378,8 @@ void launch_stage(dglm_ctx *c, const StageArgs &a) {
     if (kind == DGLM_STAGE_MFMA) {
       if (c->p.precision == DGLM_FP32)
         launch_mfma<N, PHYS, MODE, PREC_F32>(c, a);
+      else if (c->p.precision == DGLM_BF16X3 || c->p.precision == DGLM_BF16X6)
+        launch_mfma<N, PHYS, MODE, PREC_BF>(c, a);
       else
         launch_mfma<N, PHYS, MODE, PREC_F64>(c, a);
       return;
@@ -440,6 +442,7 @@ StageArgs stage_args(dglm_ctx *c, int s) {
   a.gamma = c->p.gamma;
   a.tau_scale = c->p.tau_scale;
   a.flux = c->p.flux == DGLM_FLUX_RUSANOV ? 1 : 0;
+  a.bf_terms = c->p.precision == DGLM_BF16X6 ? 6 : 3;
   a.dmap = c->nd > 0 ? c->dmap : nullptr;
   a.gbnd = c->gbnd;
   a.nd = c->nd;
@@ -563,11 +566,10 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   *out = nullptr;
   if (p->physics != DGLM_ADVECTION && p->physics != DGLM_EULER)
     return fail(DGLM_ERR_ARG, "dglm_create: physics must be DGLM_ADVECTION or DGLM_EULER");
-  if (p->precision == DGLM_BF16X3)
-    return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: the split-BF16 contraction is not built");
-  if (p->precision != DGLM_FP64 && p->precision != DGLM_FP32)
+  if (p->precision != DGLM_FP64 && p->precision != DGLM_FP32 && p->precision != DGLM_BF16X3 &&
+      p->precision != DGLM_BF16X6)
     return fail(DGLM_ERR_ARG, "dglm_create: unknown precision mode");
-  if (p->precision == DGLM_FP32 && (p->N < 1 || p->N > 6))
+  if (p->precision != DGLM_FP64 && (p->N < 1 || p->N > 6))
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_create: mixed precision is built for N in [1, 6]");
   if (p->flux != DGLM_FLUX_LM && p->flux != DGLM_FLUX_RUSANOV)
     return fail(DGLM_ERR_ARG, "dglm_create: flux must be DGLM_FLUX_LM or DGLM_FLUX_RUSANOV");
@@ -636,12 +638,19 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   // A-fragments of the stacked operator for the MFMA stage kernel.
   std::vector<double> afr;
   std::vector<float> afr32;
+  std::vector<s16x4> afrbf;
   std::vector<double> rsum;
   if (mfma_available(p->N)) {
     const int RT = (Np + 15) / 16, KS = (KD + 3) / 4;
-    if (p->precision == DGLM_FP32) {
-      afr32.resize((size_t)RT * KS * 64);
-      arrange_a_fragments(Np, KD, RT, KS, op.data(), afr32.data());
+    if (p->precision != DGLM_FP64) {
+      if (p->precision == DGLM_FP32) {
+        afr32.resize((size_t)RT * KS * 64);
+        arrange_a_fragments(Np, KD, RT, KS, op.data(), afr32.data());
+      } else {
+        const int KS16 = (KD + 15) / 16;
+        afrbf.resize((size_t)RT * KS16 * BF_PIECES * 64);
+        arrange_a_bf16(Np, KD, RT, KS16, op.data(), afrbf.data());
+      }
       // affine-offset tables: [fitV (KD x 4) | fitP (KD x 4) | fitAV (Np x NBLK x 4)]
       rsum.assign((size_t)8 * KD + (size_t)Np * NBLK * 4, 0.0);
       std::vector<double> rx(Np), ry(Np), rz(Np);
@@ -658,8 +667,9 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
       arrange_a_fragments(Np, KD, RT, KS, op.data(), afr.data());
     }
   }
-  const size_t afrag_bytes = afr.size() * sizeof(double) + afr32.size() * sizeof(float);
-  const void *afrag_src = afr.empty() ? (const void *)afr32.data() : (const void *)afr.data();
+  const size_t afrag_bytes = afr.size() * sizeof(double) + afr32.size() * sizeof(float) + afrbf.size() * sizeof(s16x4);
+  const void *afrag_src = !afr.empty() ? (const void *)afr.data()
+                                       : (!afr32.empty() ? (const void *)afr32.data() : (const void *)afrbf.data());
   // Operator rows for the warp-specialised FP64 Euler kernel.
   std::vector<double> afr_ws, arem_ws;
   // (its gathers index one variable's nodes with 32-bit offsets)

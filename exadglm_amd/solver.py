@@ -7,7 +7,7 @@ import numpy as np
 from . import _capi as A
 
 _PHYS = {"advection": A.DGLM_ADVECTION, "euler": A.DGLM_EULER}
-_PREC = {"fp64": A.DGLM_FP64, "fp32": A.DGLM_FP32, "bf16x3": A.DGLM_BF16X3}
+_PREC = {"fp64": A.DGLM_FP64, "fp32": A.DGLM_FP32, "bf16x3": A.DGLM_BF16X3, "bf16x6": A.DGLM_BF16X6}
 
 
 def make_mesh_struct(arrays, K, matrix_order="c"):
@@ -26,7 +26,9 @@ def make_mesh_struct(arrays, K, matrix_order="c"):
 class Solver:
     """Owns a device context: state, LSRK register and the step loop.
 
-    physics: 'advection' or 'euler'; precision: 'fp64' (default).
+    physics: 'advection' or 'euler'; precision: 'fp64' (default), or a mixed
+    mode for N <= 6 ('fp32', 'bf16x3', 'bf16x6': the contraction of the
+    flux remainders after a least-squares affine fit, in FP32 or split bf16).
     tmax: the run ends exactly at tmax (use math.inf for fixed step counts).
     """
 

@@ -32,8 +32,13 @@ enum {
 enum { DGLM_ADVECTION = 0, DGLM_EULER = 1 };
 
 /* Arithmetic of the operator contractions.  The LSRK state, pressure from
- * energy, wave speed and conservation sums are FP64 in every mode. */
-enum { DGLM_FP64 = 0, DGLM_FP32 = 1, DGLM_BF16X3 = 2 };
+ * energy, wave speed and conservation sums are FP64 in every mode.  The
+ * mixed modes remove a least-squares affine fit of every flux block in FP64
+ * and contract only the remainder in low precision: DGLM_FP32 on FP32 MFMA;
+ * DGLM_BF16X3 on bf16 MFMA as the split-BF16 three-term product
+ * (hi*hi + hi*lo + lo*hi of two bf16 pieces per operand); DGLM_BF16X6 with
+ * three pieces and six products.  FP32 accumulation in all three. */
+enum { DGLM_FP64 = 0, DGLM_FP32 = 1, DGLM_BF16X3 = 2, DGLM_BF16X6 = 3 };
 
 /* Numerical flux.  DGLM_FLUX_LM is the reference's Lagrange-multiplier flux
  * (n.F(ub) - tau (ub - u-)) Fscale, dim3/src/cpp/euler_dglm.cpp:110-183.
@@ -114,7 +119,7 @@ typedef struct dglm_params {
   int N;           /* polynomial order (NP, NFP follow): 1..10 in FP64,  */
                    /* 1..6 in mixed precision                           */
   int physics;     /* DGLM_ADVECTION (NVAR=1) or DGLM_EULER (NVAR=5)     */
-  int precision;   /* DGLM_FP64 / DGLM_FP32 / DGLM_BF16X3               */
+  int precision;   /* DGLM_FP64 / DGLM_FP32 / DGLM_BF16X3 / DGLM_BF16X6 */
   int flux;        /* DGLM_FLUX_LM or DGLM_FLUX_RUSANOV                  */
   double cfl;      /* CFL                                                */
   double gamma;    /* GAMMA                                              */

@@ -45,8 +45,10 @@ def test_create_rejects_bad_params():
     ctx = C.c_void_p()
     m = A.Mesh()
     m.K = 6
-    for kw, code in [(dict(physics=7), A.DGLM_ERR_ARG), (dict(precision=9), A.DGLM_ERR_ARG), (dict(precision=A.DGLM_BF16X3), A.DGLM_ERR_UNSUPPORTED),
+    for kw, code in [(dict(physics=7), A.DGLM_ERR_ARG), (dict(precision=9), A.DGLM_ERR_ARG), (dict(precision=4), A.DGLM_ERR_ARG),
                      (dict(N=11), A.DGLM_ERR_UNSUPPORTED), (dict(N=7, precision=A.DGLM_FP32), A.DGLM_ERR_UNSUPPORTED),
+                     (dict(N=7, precision=A.DGLM_BF16X3), A.DGLM_ERR_UNSUPPORTED),
+                     (dict(N=7, precision=A.DGLM_BF16X6), A.DGLM_ERR_UNSUPPORTED),
                      (dict(cfl=-1.0), A.DGLM_ERR_ARG),
                      (dict(gamma=1.0), A.DGLM_ERR_ARG)]:
         rc = L.dglm_create(C.byref(params(**kw)), C.byref(m), C.byref(ctx))

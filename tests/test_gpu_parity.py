@@ -417,10 +417,10 @@ MIXED = ["mfma_mx", "mfma"]  # the fused mixed-precision kernel (AUTO) and the g
 @pytest.mark.parametrize("flux", ["lm", "rusanov"])
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_mixed_precision_rhs(gpu, N, flux, kernel):
-    """FP32-MFMA contraction of the split fluxes (vertex-interpolation
-    remainders in the fused kernel, least-squares affine remainders in the
-    generic one): one RHS within FP32 rounding of the in-block flux variation
-    (bound 1e-7 of the term scale), both fluxes."""
+    """FP32-MFMA contraction of the split fluxes (the remainders of the
+    least-squares affine fit of every flux block, in the fused and in the
+    generic kernel): one RHS within FP32 rounding of the in-block flux
+    variation (bound 1e-7 of the term scale), both fluxes."""
     _say(f"mixed precision rhs N={N} {flux} {kernel}")
     pb = cube_problem(N, 3)
     u = smooth_state(pb, "euler", seed=N + 31)
@@ -458,6 +458,98 @@ def test_mixed_precision_vortex(gpu, N, kernel):
     uo, _, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 40)
     errs = euler_errors(u, uo)
     _say(f"  per-variable rel L2 vs oracle: {errs}")
+    assert max(errs) < 1e-8, errs
+
+
+def _term_scale(pb, u):
+    """Scale of the RHS terms: operator row sums x metric x |u|^2."""
+    a = pb.arrays
+    opn = max(np.abs(a[k]).sum(axis=1).max() for k in ("WDr", "WDs", "WDt", "LIFT"))
+    met = max(np.abs(a[k]).max() for k in ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "Fscale"))
+    return 3 * opn * met * np.abs(u).max() ** 2
+
+
+# split-BF16 contraction: (relative bound of one RHS against the term scale,
+# bound of 40 vortex steps against the oracle).  Three terms keep ~16 bits of
+# each operand (CPU emulation of the scheme at N=6, 6^3 hexes, 40 steps:
+# 3.9e-7); six terms ~24 bits (emulation with one FP32 accumulator 4.7e-9;
+# the kernel sums the cross terms in a second accumulator).
+BF16 = {"bf16x3": (3e-7, 1e-6), "bf16x6": (1e-8, 1e-8)}
+
+
+@pytest.mark.parametrize("prec", sorted(BF16))
+@pytest.mark.parametrize("flux", ["lm", "rusanov"])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_split_bf16_rhs(gpu, N, flux, prec):
+    """Split-BF16 contraction (bf16 MFMA on the least-squares affine-fit
+    remainders, FP32 accumulation, the fit's term in FP64): one RHS against
+    the FP64 oracle, both fluxes; the wave speed stays FP64."""
+    _say(f"split-bf16 rhs N={N} {flux} {prec}")
+    pb = cube_problem(N, 3)
+    u = smooth_state(pb, "euler", seed=N + 37)
+    with Solver(pb, "euler", precision=prec, flux=flux) as s:
+        assert s.stage_kernel() == "mfma"
+        r, mv, tau = s.rhs(u)
+    ro, mvo, tauo = O.Oracle(pb, "euler", flux=flux).rhs(u)
+    assert rel_err(mv, mvo) < 1e-15
+    scale = _term_scale(pb, u)
+    err = max(np.abs(r[v] - ro[v]).max() for v in range(5)) / scale
+    _say(f"  max error / term scale {err:.2e}")
+    assert err < BF16[prec][0]
+
+
+@pytest.mark.parametrize("prec", sorted(BF16))
+def test_split_bf16_advection_rhs(gpu, prec):
+    """The split-BF16 contraction for advection (N=4, LM flux)."""
+    pb = cube_problem(4, 3)
+    u = smooth_state(pb, "advection", seed=5)
+    with Solver(pb, "advection", precision=prec) as s:
+        r, _, _ = s.rhs(u)
+    ro, _, _ = O.Oracle(pb, "advection").rhs(u)
+    # linear flux, default velocity (1, 1, 1): terms scale with |u|, not |u|^2
+    scale = _term_scale(pb, u) / np.abs(u).max()
+    err = np.abs(r - ro).max() / scale
+    _say(f"  advection split-bf16 {prec}: max error / term scale {err:.2e}")
+    assert err < BF16[prec][0]
+
+
+@pytest.mark.parametrize("prec", sorted(BF16))
+@pytest.mark.parametrize("N", [4, 6])
+def test_split_bf16_vortex(gpu, N, prec):
+    """Isentropic vortex, 6^3 hexes, 40 steps with the split-BF16
+    contraction against the FP64 oracle (per-variable relative L2)."""
+    _say(f"split-bf16 vortex N={N} {prec}")
+    pb = uniform_problem(N, 7, 7, 7, box=vortex_box(6))
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    with Solver(pb, "euler", cfl=0.4, precision=prec) as s:
+        s.set_state(u0)
+        s.step(40)
+        u = s.get_state()
+    uo, _, _ = O.Oracle(pb, "euler", cfl=0.4).run(u0, 40)
+    errs = euler_errors(u, uo)
+    _say(f"  per-variable rel L2 vs oracle: {errs}")
+    assert max(errs) < BF16[prec][1], errs
+
+
+@pytest.mark.timeout(600)
+def test_c3_split_bf16_full_size(gpu):
+    """The mixed-precision configuration at full size (N=6, 82,944 tets,
+    Rusanov, 500 steps) with the six-term split-BF16 contraction: within
+    1e-8 relative L2 per conserved variable of the FP64 run (itself pinned
+    against the oracle at this size in test_c3_mixed_precision_full_size)."""
+    _say("C3 full size K=82944 N=6 500 steps (split-BF16 x6 vs FP64)")
+    pb = uniform_problem(6, 25, 25, 25, box=vortex_box(24))
+    a = pb.arrays
+    u0 = isentropic_vortex(a["px"], a["py"], a["pz"])
+    out = {}
+    for prec in ("fp64", "bf16x6"):
+        with Solver(pb, "euler", cfl=0.4, flux="rusanov", precision=prec) as s:
+            s.set_state(u0)
+            s.step(500)
+            out[prec] = s.get_state()
+    errs = euler_errors(out["bf16x6"], out["fp64"])
+    _say(f"  split-BF16 x6 vs FP64 after 500 steps: {errs}")
     assert max(errs) < 1e-8, errs
 
 
