@@ -131,6 +131,9 @@ inline void mx_build_tables(const double *opT, const double *fitV, const double 
       }
 }
 
+// Two doubles at an 8-byte aligned address: one 16-byte memory instruction.
+typedef double f64x2u __attribute__((ext_vector_type(2), aligned(8)));
+
 // Workgroup barrier over LDS: waits for this wave's LDS (and scalar memory)
 // operations only, not for its global loads and stores, so those stay in
 // flight across it (no wave of the workgroup reads global memory another wave
@@ -163,6 +166,38 @@ __device__ inline void mx_surface_flux5(const double uM[5], const double uP[5], 
   nF[4] = (ub[4] + p) * vn;
 #pragma unroll
   for (int v = 0; v < 5; v++) fl[v] = (nF[v] - tau * (ub[v] - uM[v])) * g[3];
+}
+
+// Cross-lane sums without the LDS crossbar: a double moved by one DPP
+// permutation (two 32-bit moves), and all-reduce sums over aligned groups of
+// 8 lanes (quad swaps, then the half-row mirror) and of 32 lanes (rotations
+// within a 16-lane row, then a swizzle with xor 16).  Every lane of a group
+// ends with the same sum, formed in a fixed order (deterministic).
+template <int CTRL>
+__device__ inline double dpp_f64(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xF, 0xF, false);
+  const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline double swz_xor16_f64(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  constexpr int PAT = (0x10 << 10) | 0x1F;  // bit mode: and 0x1f, or 0, xor 0x10
+  const unsigned lo = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)u, PAT);
+  const unsigned hi = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)(u >> 32), PAT);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ inline double sum8(double x) {
+  x += dpp_f64<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E>(x);   // quad_perm [2,3,0,1]
+  return x + dpp_f64<0x141>(x);  // row_half_mirror: the other quad of the 8
+}
+__device__ inline double sum32(double x) {
+  x += dpp_f64<0x128>(x);  // row_ror 8
+  x += dpp_f64<0x124>(x);  // row_ror 4
+  x += dpp_f64<0x122>(x);  // row_ror 2
+  x += dpp_f64<0x121>(x);  // row_ror 1
+  return x + swz_xor16_f64(x);
 }
 
 // Volume fluxes (fR, fS, fT) of variable v at one node: g_d . F with the
@@ -221,9 +256,7 @@ __device__ inline void mx_volume_pass(int v, int w, int lane, int ne, const doub
     }
   }
 #pragma unroll
-  for (int o = 1; o < 32; o <<= 1)
-#pragma unroll
-    for (int q = 0; q < 12; q++) c[q] += __shfl_xor(c[q], o, 64);
+  for (int q = 0; q < 12; q++) c[q] = sum32(c[q]);
   if (sl == 0)
 #pragma unroll
     for (int q = 0; q < 12; q++) cv[q * EB + el] = c[q];
@@ -253,9 +286,9 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
   constexpr int Np = S::Np, Nfp = S::Nfp, NFF = S::NFF, KS = S::KS, KSV = S::KSV, KV4 = S::KV4;
   constexpr int NT = S::NT, RT = S::RT, EB = S::EB, NC = S::NC, UST = S::UST;
   constexpr int JF = (Nfp + 7) / 8;              // face nodes per thread in the surface pass
-  constexpr int NLD = (EB * Np + NT - 1) / NT;   // node values per thread and variable in the staging
   static_assert(S::NW * 2 == EB && RT <= S::NW, "a volume pass takes two elements per wave");
   static_assert(NT == 4 * EB * 8, "the surface pass takes a thread per (face, group of eight)");
+  static_assert(S::NW > RT && EB * 4 <= 64, "a helper wave stages the connectivity");
   const float *afrag = (const float *)(tab + TB::afr_off);
   const double *avfrag = (const double *)(tab + TB::avf_off);
 
@@ -305,36 +338,55 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
   const int lo = (int)((long)nbatch * grp / ngrp), hi = (int)((long)nbatch * (grp + 1) / ngrp);
 
   // Stages 1-3 of a batch, run by every wave (each role calls it at the same
-  // point of its own loop; both roles pass the same barriers).
-  auto stage_in = [&](int e0, int ne) {
+  // point of its own loop; both roles pass the same barriers).  `helper` is
+  // a constant at each call, so each role's copy holds only its own registers.
+  auto stage_in = [&](int e0, int ne, const bool helper) {
     mx_barrier();  // the previous batch is done with every LDS buffer
     // ---- 1. node values (columns past the last element: a state at rest),
-    //      geometry, connectivity, tau; the loads issued together ----
-    {
-      double ld[5][NLD];
+    //      geometry, connectivity, tau.  Only the helper waves load, every
+    //      value of the batch into registers at once (one round trip), then
+    //      into LDS; the contraction waves, whose registers hold their
+    //      operator rows, issue no loads here (register loads there had to
+    //      spill, and direct-to-LDS copies issued by two waves were slower) ----
+    if (helper) {
+      const int hw = w - RT, hl = tid - 64 * RT, HT = 64 * (S::NW - RT);
+      constexpr int NHX = (EB * Np + 64 * (S::NW - RT) - 1) / (64 * (S::NW - RT));  // node values per lane and variable
+      constexpr int NHG = (EB * GEO_STRIDE + 64 * (S::NW - RT) - 1) / (64 * (S::NW - RT));
+      double ld[5][NHX], lg[NHG];
 #pragma unroll
-      for (int v = 0; v < 5; v++)
+      for (int v = 0; v < 5; v++) {
+        const double *bv = a.u_in + v * KNp + (size_t)e0 * Np;
 #pragma unroll
-        for (int x = 0; x < NLD; x++) {
-          const int i = tid + NT * x;
-          ld[v][x] = i < ne * Np ? a.u_in[v * KNp + (size_t)e0 * Np + i] : (v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0));
+        for (int x = 0; x < NHX; x++) {
+          const int i = hl + HT * x;
+          ld[v][x] = i < ne * Np ? bv[i] : (v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0));
         }
+      }
 #pragma unroll
-      for (int x = 0; x < NLD; x++) {
-        const int i = tid + NT * x, e = i / Np, j = i - e * Np;
+      for (int x = 0; x < NHG; x++) {
+        const int i = hl + HT * x;
+        lg[x] = i < ne * GEO_STRIDE ? a.geo[(size_t)e0 * GEO_STRIDE + i] : (i % GEO_STRIDE < 9 ? 0.0 : 1.0);
+      }
+#pragma unroll
+      for (int x = 0; x < NHX; x++) {
+        const int i = hl + HT * x, e = i / Np, j = i - e * Np;
         if (i < EB * Np)
 #pragma unroll
           for (int v = 0; v < 5; v++) s_u[v][e * UST + j] = ld[v][x];
       }
-    }
-    for (int i = tid; i < EB * GEO_STRIDE; i += NT)
-      s_geo[i] = i < ne * GEO_STRIDE ? a.geo[(size_t)e0 * GEO_STRIDE + i] : (i % GEO_STRIDE < 9 ? 0.0 : 1.0);
-    if (tid < EB * 4) {
-      const bool live = tid < ne * 4;
-      const int c = live ? a.conn[(size_t)e0 * 4 + tid] : 0;
-      s_conn[tid] = c;
-      s_tau[tid] = live ? 0.25 * (a.maxv[(size_t)e0 * 4 + tid] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) * a.tau_scale
-                        : 0.0;
+#pragma unroll
+      for (int x = 0; x < NHG; x++) {
+        const int i = hl + HT * x;
+        if (i < EB * GEO_STRIDE) s_geo[i] = lg[x];
+      }
+      if (hw == 0) {
+        const bool live = lane < ne * 4;
+        const int c = live ? a.conn[(size_t)e0 * 4 + lane] : 0;
+        s_conn[lane] = c;
+        s_tau[lane] = live ? 0.25 * (a.maxv[(size_t)e0 * 4 + lane] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) *
+                                 a.tau_scale
+                           : 0.0;
+      }
     }
     mx_barrier();
     // ---- 2. surface fluxes of all five variables at every face-node pair,
@@ -385,11 +437,9 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
         }
       }
 #pragma unroll
-      for (int o = 1; o < 8; o <<= 1)
+      for (int v = 0; v < 5; v++)
 #pragma unroll
-        for (int v = 0; v < 5; v++)
-#pragma unroll
-          for (int qq = 0; qq < 3; qq++) c[v][qq] += __shfl_xor(c[v][qq], o, 64);
+        for (int qq = 0; qq < 3; qq++) c[v][qq] = sum8(c[v][qq]);
       if (s8 == 0)
 #pragma unroll
         for (int v = 0; v < 5; v++)
@@ -418,7 +468,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
     for (int bq = lo + gsl; bq < hi; bq += gsz) {
       const int b = order ? order[bq] : bq;
       const int e0 = b * EB, ne = min(EB, K - e0);
-      stage_in(e0, ne);
+      stage_in(e0, ne, true);
 #pragma unroll 1
       for (int v = 0; v < 5; v++) {
         if (v < 4)
@@ -442,19 +492,34 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
   for (int bq = lo + gsl; bq < hi; bq += gsz) {
     const int b = order ? order[bq] : bq;
     const int e0 = b * EB, ne = min(EB, K - e0);
-    stage_in(e0, ne);
+    stage_in(e0, ne, false);
     // ---- 4. per variable: the contraction, the fit term, the update, then
     //      this wave's share of the next variable's volume pass ----
 #pragma unroll 1
     for (int v = 0; v < 5; v++) {
-      // the LSRK register of rows 16w + 4kq + r, loaded ahead of the chain
+      // the LSRK register of rows i0 + r, loaded ahead of the chain; a lane's
+      // four rows are consecutive in memory, so whole rows go as two 16-byte
+      // accesses (8-byte accesses scattered over 16 elements x 4 row groups
+      // cost a memory transaction per lane, and every later wait of this
+      // wave on a load also waits for its stores to drain)
+      const int i0 = 16 * w + 4 * kq;
+      const size_t g0 = v * KNp + (size_t)(e0 + ec) * Np + i0;
+      const bool full = i0 + 3 < Np;
       double kold[4] = {0.0, 0.0, 0.0, 0.0};
-      if (MODE == 0 && ec < ne)
+      if (MODE == 0 && ec < ne) {
+        if (full) {
+          const f64x2u k01 = *reinterpret_cast<const f64x2u *>(a.k + g0);
+          const f64x2u k23 = *reinterpret_cast<const f64x2u *>(a.k + g0 + 2);
+          kold[0] = k01[0];
+          kold[1] = k01[1];
+          kold[2] = k23[0];
+          kold[3] = k23[1];
+        } else {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = 16 * w + 4 * kq + r;
-          if (i < Np) kold[r] = a.k[v * KNp + (size_t)(e0 + ec) * Np + i];
+          for (int r = 0; r < 4; r++)
+            if (i0 + r < Np) kold[r] = a.k[g0 + r];
         }
+      }
       const float *vr = s_vr[v & 1];
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       // (a scheduling fence every 8 k-steps bounds the LDS reads hoisted
@@ -470,20 +535,33 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
       for (int ks = 0; ks < NC / 4; ks++)
         acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(avr[ks], s_c[v][(4 * ks + kq) * EB + ec], acc64, 0, 0, 0);
       if (ec < ne) {
+        double o1[4], o2[4];  // MODE 0: k and u_out; MODE 1: rhs
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int i = 16 * w + 4 * kq + r;
-          if (i < Np) {
-            const double rhs = (double)acc[r] + acc64[r];
-            const size_t gi = v * KNp + (size_t)(e0 + ec) * Np + i;
-            if (MODE == 1) {
-              a.rhs[gi] = rhs;
-            } else {
-              const double kn = a.rka * kold[r] + dt * rhs;
-              a.k[gi] = kn;
-              a.u_out[gi] = s_u[v][ec * UST + i] + a.rkb * kn;
-            }
+          const double rhs = (double)acc[r] + acc64[r];
+          if (MODE == 1) {
+            o1[r] = rhs;
+            o2[r] = 0.0;
+          } else {
+            o1[r] = a.rka * kold[r] + dt * rhs;
+            o2[r] = s_u[v][ec * UST + min(i0 + r, Np - 1)] + a.rkb * o1[r];
           }
+        }
+        double *d1 = MODE == 1 ? a.rhs : a.k;
+        if (full) {
+          *reinterpret_cast<f64x2u *>(d1 + g0) = f64x2u{o1[0], o1[1]};
+          *reinterpret_cast<f64x2u *>(d1 + g0 + 2) = f64x2u{o1[2], o1[3]};
+          if (MODE == 0) {
+            *reinterpret_cast<f64x2u *>(a.u_out + g0) = f64x2u{o2[0], o2[1]};
+            *reinterpret_cast<f64x2u *>(a.u_out + g0 + 2) = f64x2u{o2[2], o2[3]};
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            if (i0 + r < Np) {
+              d1[g0 + r] = o1[r];
+              if (MODE == 0) a.u_out[g0 + r] = o2[r];
+            }
         }
       }
       if (v < 4)
