@@ -286,13 +286,16 @@ inline bool ws_built(int N) {
 }
 
 // The stage kernel that runs: the request where it exists, else the best
-// available (the mixed-precision contraction exists only as an MFMA kernel).
+// available: the warp-specialised kernel (FP64, and DGLM_FP32 at N = 3, 4),
+// else the fused mixed-precision kernel (DGLM_FP32 Euler, N <= 6), else the
+// generic MFMA kernel, else the LDS-staged one.
 int stage_kind(const dglm_ctx *c) {
   const bool ws = c->afrag_ws != nullptr, mf = mfma_available(c->p.N) && c->afrag != nullptr;
   if (c->variant == DGLM_STAGE_SIMPLE && c->p.precision == DGLM_FP64) return DGLM_STAGE_SIMPLE;
   if (c->variant == DGLM_STAGE_MFMA && mf) return DGLM_STAGE_MFMA;
-  if (c->mx_tab) return DGLM_STAGE_MFMA_MX;
+  if (c->variant == DGLM_STAGE_MFMA_MX && c->mx_tab) return DGLM_STAGE_MFMA_MX;
   if (ws) return DGLM_STAGE_MFMA_WS;
+  if (c->mx_tab) return DGLM_STAGE_MFMA_MX;
   return mf ? DGLM_STAGE_MFMA : DGLM_STAGE_SIMPLE;
 }
 
@@ -670,10 +673,15 @@ extern "C" int dglm_create(const dglm_params *p, const dglm_mesh *m, dglm_ctx **
   const size_t afrag_bytes = afr.size() * sizeof(double) + afr32.size() * sizeof(float) + afrbf.size() * sizeof(s16x4);
   const void *afrag_src = !afr.empty() ? (const void *)afr.data()
                                        : (!afr32.empty() ? (const void *)afr32.data() : (const void *)afrbf.data());
-  // Operator rows for the warp-specialised FP64 Euler kernel.
+  // Operator rows for the warp-specialised FP64 Euler kernel, which also runs
+  // DGLM_FP32 at the orders it is built for (N = 3, 4): there its FP64
+  // contraction outruns the fused mixed-precision kernel, whose least-squares
+  // fits cost more FP64 work than the FP32 MFMA saves when the stage is bound
+  // by the FP64 flux arithmetic, and it is exact where FP32 promises 1e-8.
   std::vector<double> afr_ws, arem_ws;
   // (its gathers index one variable's nodes with 32-bit offsets)
-  if (p->physics == DGLM_EULER && p->precision == DGLM_FP64 && ws_built(p->N) && (long long)K * Np < (1LL << 31)) {
+  if (p->physics == DGLM_EULER && (p->precision == DGLM_FP64 || p->precision == DGLM_FP32) && ws_built(p->N) &&
+      (long long)K * Np < (1LL << 31)) {
     if (p->N == 3) {
       afr_ws.resize((size_t)WsShape<3>::RTF * WsShape<3>::KS * 64);
       arem_ws.resize((size_t)WsShape<3>::REM * WsShape<3>::KDP);
@@ -1166,7 +1174,8 @@ extern "C" int dglm_set_stage_kernel(dglm_ctx *c, int variant) {
                                       "precision DGLM_FP32, N <= 6 and no Dirichlet faces");
   if (variant == DGLM_STAGE_MFMA_WS && !c->afrag_ws)
     return fail(DGLM_ERR_UNSUPPORTED,
-                "dglm_set_stage_kernel: the warp-specialised kernel is FP64 Euler with N = 3 or 4");
+                "dglm_set_stage_kernel: the warp-specialised kernel is Euler with N = 3 or 4 and precision "
+                "DGLM_FP64 or DGLM_FP32");
   if (variant == DGLM_STAGE_MFMA && !mfma_available(c->p.N))
     return fail(DGLM_ERR_UNSUPPORTED, "dglm_set_stage_kernel: no MFMA kernel for this order");
   if (variant == DGLM_STAGE_SIMPLE && c->p.precision != DGLM_FP64)

@@ -37,7 +37,12 @@ enum { DGLM_ADVECTION = 0, DGLM_EULER = 1 };
  * and contract only the remainder in low precision: DGLM_FP32 on FP32 MFMA;
  * DGLM_BF16X3 on bf16 MFMA as the split-BF16 three-term product
  * (hi*hi + hi*lo + lo*hi of two bf16 pieces per operand); DGLM_BF16X6 with
- * three pieces and six products.  FP32 accumulation in all three. */
+ * three pieces and six products.  FP32 accumulation in all three.  Each
+ * mode promises a result within its bound of FP64 (DGLM_FP32 and
+ * DGLM_BF16X6 1e-8 relative), not a particular kernel: at Euler N = 3, 4
+ * DGLM_FP32 runs the FP64 warp-specialised kernel, which is faster there than
+ * the mixed-precision one (dglm_set_stage_kernel(DGLM_STAGE_MFMA_MX) still
+ * selects the latter). */
 enum { DGLM_FP64 = 0, DGLM_FP32 = 1, DGLM_BF16X3 = 2, DGLM_BF16X6 = 3 };
 
 /* Numerical flux.  DGLM_FLUX_LM is the reference's Lagrange-multiplier flux
@@ -237,9 +242,11 @@ int dglm_set_timing(dglm_ctx *ctx, int on);
  * MFMA: one workgroup per 16-element batch, operator row tiles in registers
  * (N <= 6, FP64 and mixed precision); MFMA_WS: persistent warp-specialised
  * FP64 Euler kernel (producer waves stage the next batch while consumer waves
- * run the MFMA contraction; N = 3, 4).  AUTO takes MFMA_WS where it exists,
- * else MFMA, else SIMPLE (N = 7..10).  dglm_get_stage_kernel reports the
- * kernel that runs. */
+ * run the MFMA contraction; N = 3, 4, precision DGLM_FP64 or DGLM_FP32);
+ * MFMA_MX: fused mixed-precision Euler kernel (DGLM_FP32, N <= 6, no
+ * Dirichlet faces).  AUTO takes MFMA_WS where it exists, else MFMA_MX, else
+ * MFMA, else SIMPLE (N = 7..10).  dglm_get_stage_kernel reports the kernel
+ * that runs. */
 enum { DGLM_STAGE_AUTO = 0, DGLM_STAGE_SIMPLE = 1, DGLM_STAGE_MFMA = 2, DGLM_STAGE_MFMA_WS = 3, DGLM_STAGE_MFMA_MX = 4 };
 int dglm_set_stage_kernel(dglm_ctx *ctx, int variant);
 int dglm_get_stage_kernel(dglm_ctx *ctx, int *variant);
