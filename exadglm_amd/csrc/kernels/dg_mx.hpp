@@ -337,36 +337,56 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
   const int grp = blockIdx.x % ngrp, gsl = blockIdx.x / ngrp, gsz = gridDim.x / ngrp;
   const int lo = (int)((long)nbatch * grp / ngrp), hi = (int)((long)nbatch * (grp + 1) / ngrp);
 
+  // Staging values of a batch (node values, geometry, connectivity, tau),
+  // loaded by the helper waves only: the contraction waves' registers hold
+  // their operator rows (register loads there had to spill, and direct-to-LDS
+  // copies issued by two waves were slower).  The helpers load the next
+  // batch's values during the current batch's variable loop (the
+  // connectivity after its first volume pass, the rest after its last one,
+  // so that no volume pass holds them), and only their LDS writes remain at
+  // the next stage-in.  Columns past the last element hold a state at rest.
+  constexpr int NHW = S::NW - RT;                                      // helper waves
+  constexpr int NHX = (EB * Np + 64 * NHW - 1) / (64 * NHW);           // node values per lane and variable
+  constexpr int NHG = (EB * GEO_STRIDE + 64 * NHW - 1) / (64 * NHW);  // geometry values per lane
+  const int hl = tid - 64 * RT, HT = 64 * NHW;                         // helper thread index and count
+  double ld[5][NHX], lg[NHG];
+  int lc = 0;       // helper wave 0: connectivity word of face `lane` of the batch
+  double lt = 0.0;  // ... and its tau
+  auto fetch = [&](int e0, int ne) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const double *bv = a.u_in + v * KNp + (size_t)e0 * Np;
+#pragma unroll
+      for (int x = 0; x < NHX; x++) {
+        const int i = hl + HT * x;
+        ld[v][x] = i < ne * Np ? bv[i] : (v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0));
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < NHG; x++) {
+      const int i = hl + HT * x;
+      lg[x] = i < ne * GEO_STRIDE ? a.geo[(size_t)e0 * GEO_STRIDE + i] : (i % GEO_STRIDE < 9 ? 0.0 : 1.0);
+    }
+  };
+  auto fetch_conn = [&](int e0, int ne) {
+    if (w == RT) lc = lane < ne * 4 ? a.conn[(size_t)e0 * 4 + lane] : 0;
+  };
+  // tau of the batch's faces (reads the connectivity word: called well after
+  // fetch_conn)
+  auto fetch_tau = [&](int e0, int ne) {
+    if (w == RT)
+      lt = lane < ne * 4 ? 0.25 * (a.maxv[(size_t)e0 * 4 + lane] + a.maxv[conn_nbr(lc) * 4 + conn_face(lc)]) *
+                               a.tau_scale
+                         : 0.0;
+  };
+
   // Stages 1-3 of a batch, run by every wave (each role calls it at the same
   // point of its own loop; both roles pass the same barriers).  `helper` is
   // a constant at each call, so each role's copy holds only its own registers.
   auto stage_in = [&](int e0, int ne, const bool helper) {
     mx_barrier();  // the previous batch is done with every LDS buffer
-    // ---- 1. node values (columns past the last element: a state at rest),
-    //      geometry, connectivity, tau.  Only the helper waves load, every
-    //      value of the batch into registers at once (one round trip), then
-    //      into LDS; the contraction waves, whose registers hold their
-    //      operator rows, issue no loads here (register loads there had to
-    //      spill, and direct-to-LDS copies issued by two waves were slower) ----
+    // ---- 1. the helpers' fetched values into LDS ----
     if (helper) {
-      const int hw = w - RT, hl = tid - 64 * RT, HT = 64 * (S::NW - RT);
-      constexpr int NHX = (EB * Np + 64 * (S::NW - RT) - 1) / (64 * (S::NW - RT));  // node values per lane and variable
-      constexpr int NHG = (EB * GEO_STRIDE + 64 * (S::NW - RT) - 1) / (64 * (S::NW - RT));
-      double ld[5][NHX], lg[NHG];
-#pragma unroll
-      for (int v = 0; v < 5; v++) {
-        const double *bv = a.u_in + v * KNp + (size_t)e0 * Np;
-#pragma unroll
-        for (int x = 0; x < NHX; x++) {
-          const int i = hl + HT * x;
-          ld[v][x] = i < ne * Np ? bv[i] : (v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0));
-        }
-      }
-#pragma unroll
-      for (int x = 0; x < NHG; x++) {
-        const int i = hl + HT * x;
-        lg[x] = i < ne * GEO_STRIDE ? a.geo[(size_t)e0 * GEO_STRIDE + i] : (i % GEO_STRIDE < 9 ? 0.0 : 1.0);
-      }
 #pragma unroll
       for (int x = 0; x < NHX; x++) {
         const int i = hl + HT * x, e = i / Np, j = i - e * Np;
@@ -379,13 +399,9 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
         const int i = hl + HT * x;
         if (i < EB * GEO_STRIDE) s_geo[i] = lg[x];
       }
-      if (hw == 0) {
-        const bool live = lane < ne * 4;
-        const int c = live ? a.conn[(size_t)e0 * 4 + lane] : 0;
-        s_conn[lane] = c;
-        s_tau[lane] = live ? 0.25 * (a.maxv[(size_t)e0 * 4 + lane] + a.maxv[conn_nbr(c) * 4 + conn_face(c)]) *
-                                 a.tau_scale
-                           : 0.0;
+      if (w == RT) {
+        s_conn[lane] = lc;
+        s_tau[lane] = lt;
       }
     }
     mx_barrier();
@@ -464,17 +480,33 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
   };
 
   if (!cw) {
-    // ---- helper waves: their share of the stages and of every volume pass ----
+    // ---- helper waves: their share of the stages and of every volume pass,
+    //      and the next batch's staging loads ----
+    if (lo + gsl < hi) {
+      const int b = order ? order[lo + gsl] : lo + gsl;
+      const int e0 = b * EB, ne = min(EB, K - e0);
+      fetch_conn(e0, ne);
+      fetch(e0, ne);
+      fetch_tau(e0, ne);
+    }
     for (int bq = lo + gsl; bq < hi; bq += gsz) {
       const int b = order ? order[bq] : bq;
       const int e0 = b * EB, ne = min(EB, K - e0);
       stage_in(e0, ne, true);
+      // the next batch (after the last one, this one again: the fetches are
+      // unconditional, so that no path carries their registers through the
+      // volume passes)
+      const int bn = bq + gsz < hi ? (order ? order[bq + gsz] : bq + gsz) : b;
+      const int e1 = bn * EB, ne1 = min(EB, K - e1);
 #pragma unroll 1
-      for (int v = 0; v < 5; v++) {
-        if (v < 4)
-          mx_volume_pass<N>(v + 1, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
+      for (int v = 0; v < 4; v++) {
+        mx_volume_pass<N>(v + 1, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
+        if (v == 0) fetch_conn(e1, ne1);
         mx_barrier();
       }
+      fetch(e1, ne1);
+      fetch_tau(e1, ne1);
+      mx_barrier();  // the contraction waves' last variable
     }
     return;
   }
