@@ -372,7 +372,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
     if (w == RT) lc = lane < ne * 4 ? a.conn[(size_t)e0 * 4 + lane] : 0;
   };
   // tau of the batch's faces (reads the connectivity word: called well after
-  // fetch_conn)
+  // fetch_conn, and before fetch)
   auto fetch_tau = [&](int e0, int ne) {
     if (w == RT)
       lt = lane < ne * 4 ? 0.25 * (a.maxv[(size_t)e0 * 4 + lane] + a.maxv[conn_nbr(lc) * 4 + conn_face(lc)]) *
@@ -486,8 +486,8 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
       const int b = order ? order[lo + gsl] : lo + gsl;
       const int e0 = b * EB, ne = min(EB, K - e0);
       fetch_conn(e0, ne);
-      fetch(e0, ne);
       fetch_tau(e0, ne);
+      fetch(e0, ne);
     }
     for (int bq = lo + gsl; bq < hi; bq += gsz) {
       const int b = order ? order[bq] : bq;
@@ -504,8 +504,10 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
         if (v == 0) fetch_conn(e1, ne1);
         mx_barrier();
       }
-      fetch(e1, ne1);
+      // (tau first: its wait for the connectivity word would also wait for
+      // the node-value loads issued before it)
       fetch_tau(e1, ne1);
+      fetch(e1, ne1);
       mx_barrier();  // the contraction waves' last variable
     }
     return;
