@@ -300,6 +300,11 @@ def test_c5_euler_adjoint_full_size(gpu):
     for prec in ("fp64", "fp32"):
         _say(f"C5 Euler adjoint K={pb.K} N=4 {steps} steps, {prec} forward")
         with Solver(pb, "euler", cfl=w["cfl"], precision=prec) as s:
+            if prec == "fp32":
+                # the FP32-MFMA forward is the fused mixed kernel, selected
+                # explicitly (AUTO runs the faster FP64 kernel at N = 4)
+                assert s.stage_kernel() == "mfma_ws"
+                s.set_stage_kernel("mfma_mx")
             eps = torch.tensor(w["bump"], device=dev, requires_grad=True)
             J = objective(s, eps)
             J.backward()
@@ -325,7 +330,7 @@ def test_c5_euler_adjoint_full_size(gpu):
                         assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
     e = rel_err(grads["fp32"], grads["fp64"])
     _say(f"  FP32-forward gradient vs FP64: {e:.3e}")
-    assert e < 1e-5
+    assert 0 < e < 1e-5  # a different forward, close to the FP64 one
 
 
 @pytest.mark.parametrize("nsteps", [20, 21])
