@@ -55,6 +55,8 @@ struct MxShape {
   static constexpr int KV4 = 4 * KSV;
   static constexpr int KS = KSV + Nfp;  // + surface k-steps (NFF = 4 Nfp rows)
   static constexpr int EB = 16;
+  static constexpr int RS = EB + 1;  // row stride of the FP32 remainder tiles (odd: the volume pass writes
+                                     // one element at 32 consecutive rows, which then spread over the banks)
   static constexpr int NC = 24;  // fit coefficients per (variable, element): 3 x 4 volume + 4 x 3 face
   static constexpr int UST = Np + 1;  // element stride of the staged node values (odd: a 16-element
                                       // column spreads over the LDS banks)
@@ -227,7 +229,8 @@ __device__ inline void mx_volume_flux1(int v, const double u[5], const double *g
 // elements 2w, 2w+1 of the batch, their least-squares fits c (FP64, into cv)
 // and the FP32 remainders into vr [KV4][EB].  Lane (element el, group sl)
 // takes nodes j = sl + 32x; the fit sums meet across the 32 groups in
-// registers.
+// registers.  pv, vv: fit weights and basis as [q][Np] (a read of one q by
+// 32 consecutive nodes spreads over the LDS banks; [Np][4] put them on 8).
 template <int N>
 __device__ inline void mx_volume_pass(int v, int w, int lane, int ne, const double *s_u, const double *s_geo,
                                       const double *pv, const double *vv, double gamma, double *cv, float *vr) {
@@ -248,11 +251,10 @@ __device__ inline void mx_volume_pass(int v, int w, int lane, int ne, const doub
 #pragma unroll
       for (int q = 0; q < 5; q++) u[q] = s_u[q * EB * UST + el * UST + j];
       mx_volume_flux1(v, u, gm, gamma, f[x]);
-      const double *P = &pv[j * 4];
 #pragma unroll
       for (int d = 0; d < 3; d++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) c[d * 4 + q] += P[q] * f[x][d];
+        for (int q = 0; q < 4; q++) c[d * 4 + q] += pv[q * Np + j] * f[x][d];
     }
   }
 #pragma unroll
@@ -264,11 +266,11 @@ __device__ inline void mx_volume_pass(int v, int w, int lane, int ne, const doub
   for (int x = 0; x < JX; x++) {
     const int j = sl + 32 * x;
     if (j < Np) {
-      const double *V = &vv[j * 4];
+      const double V[4] = {vv[j], vv[Np + j], vv[2 * Np + j], vv[3 * Np + j]};
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         const double fit = V[0] * c[d * 4] + V[1] * c[d * 4 + 1] + V[2] * c[d * 4 + 2] + V[3] * c[d * 4 + 3];
-        vr[(d * Np + j) * EB + el] = el < ne ? (float)(f[x][d] - fit) : 0.0f;
+        vr[(d * Np + j) * S::RS + el] = el < ne ? (float)(f[x][d] - fit) : 0.0f;
       }
     }
   }
@@ -293,10 +295,10 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
   const double *avfrag = (const double *)(tab + TB::avf_off);
 
   __shared__ double s_u[5][EB * UST];     // node values of the batch [v][e*UST + j]
-  __shared__ float s_vr[2][KV4 * EB];     // volume remainders of two variables [k][e]
-  __shared__ float s_sr[5][NFF * EB];     // surface remainders of all variables [v][m][e]
+  __shared__ float s_vr[2][KV4 * S::RS];  // volume remainders of two variables [k][e]
+  __shared__ float s_sr[5][NFF * S::RS];  // surface remainders of all variables [v][m][e]
   __shared__ double s_c[5][NC * EB];      // fit coefficients [v][q'][e]
-  __shared__ double s_tabv[2][Np * 4];    // fit weights and basis of the volume blocks
+  __shared__ double s_tabv[2][Np * 4];    // fit weights and basis of the volume blocks, [q][Np]
   __shared__ double s_tabf[2][NFF * 4];   // ... and of the faces
   __shared__ double s_geo[EB * GEO_STRIDE];
   __shared__ double s_tau[EB * 4];
@@ -323,15 +325,15 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
     const double *pv = (const double *)(tab + TB::pv_off), *vv = (const double *)(tab + TB::vv_off);
     const double *pf = (const double *)(tab + TB::pf_off), *vf = (const double *)(tab + TB::vf_off);
     for (int i = tid; i < Np * 4; i += NT) {
-      s_tabv[0][i] = pv[i];
-      s_tabv[1][i] = vv[i];
+      s_tabv[0][(i & 3) * Np + (i >> 2)] = pv[i];
+      s_tabv[1][(i & 3) * Np + (i >> 2)] = vv[i];
     }
     for (int i = tid; i < NFF * 4; i += NT) {
       s_tabf[0][i] = pf[i];
       s_tabf[1][i] = vf[i];
     }
   }
-  for (int i = 3 * Np * EB + tid; i < KV4 * EB; i += NT) s_vr[0][i] = s_vr[1][i] = 0.0f;  // padded rows stay zero
+  for (int i = 3 * Np * S::RS + tid; i < KV4 * S::RS; i += NT) s_vr[0][i] = s_vr[1][i] = 0.0f;  // padded rows stay zero
   const double *pv = s_tabv[0], *vv = s_tabv[1], *pf = s_tabf[0], *vf = s_tabf[1];
 
   const int grp = blockIdx.x % ngrp, gsl = blockIdx.x / ngrp, gsz = gridDim.x / ngrp;
@@ -469,7 +471,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
           const double *V = &vf[m * 4];
 #pragma unroll
           for (int v = 0; v < 5; v++)
-            s_sr[v][m * EB + e] =
+            s_sr[v][m * S::RS + e] =
                 e < ne ? (float)(fl[x][v] - (V[0] * c[v][0] + V[1] * c[v][1] + V[2] * c[v][2])) : 0.0f;
         }
       }
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
       // ahead of the chain, and with them the live registers)
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) {
-        const float bf = ks < KSV ? vr[(4 * ks + kq) * EB + ec] : s_sr[v][(4 * (ks - KSV) + kq) * EB + ec];
+        const float bf = ks < KSV ? vr[(4 * ks + kq) * S::RS + ec] : s_sr[v][(4 * (ks - KSV) + kq) * S::RS + ec];
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks], bf, acc, 0, 0, 0);
         if (ks % 8 == 7) __builtin_amdgcn_sched_barrier(0);
       }
