@@ -55,8 +55,10 @@ struct MxShape {
   static constexpr int KV4 = 4 * KSV;
   static constexpr int KS = KSV + Nfp;  // + surface k-steps (NFF = 4 Nfp rows)
   static constexpr int EB = 16;
-  static constexpr int RS = EB + 1;  // row stride of the FP32 remainder tiles (odd: the volume pass writes
-                                     // one element at 32 consecutive rows, which then spread over the banks)
+  static constexpr int RS = EB + 1;  // row stride of the FP32 volume remainder tile (odd: a volume pass
+                                     // writes one element at 32 consecutive rows, which then spread over
+                                     // the banks); the surface tile keeps EB (its reads are then
+                                     // conflict-free and its writes, 8 elements at 8 face rows, 2-way)
   static constexpr int NC = 24;  // fit coefficients per (variable, element): 3 x 4 volume + 4 x 3 face
   static constexpr int UST = Np + 1;  // element stride of the staged node values (odd: a 16-element
                                       // column spreads over the LDS banks)
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
 
   __shared__ double s_u[5][EB * UST];     // node values of the batch [v][e*UST + j]
   __shared__ float s_vr[2][KV4 * S::RS];  // volume remainders of two variables [k][e]
-  __shared__ float s_sr[5][NFF * S::RS];  // surface remainders of all variables [v][m][e]
+  __shared__ float s_sr[5][NFF * EB];     // surface remainders of all variables [v][m][e]
   __shared__ double s_c[5][NC * EB];      // fit coefficients [v][q'][e]
   __shared__ double s_tabv[2][Np * 4];    // fit weights and basis of the volume blocks, [q][Np]
   __shared__ double s_tabf[2][NFF * 4];   // ... and of the faces
@@ -471,7 +473,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
           const double *V = &vf[m * 4];
 #pragma unroll
           for (int v = 0; v < 5; v++)
-            s_sr[v][m * S::RS + e] =
+            s_sr[v][m * EB + e] =
                 e < ne ? (float)(fl[x][v] - (V[0] * c[v][0] + V[1] * c[v][1] + V[2] * c[v][2])) : 0.0f;
         }
       }
@@ -562,7 +564,7 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
       // ahead of the chain, and with them the live registers)
 #pragma unroll
       for (int ks = 0; ks < KS; ks++) {
-        const float bf = ks < KSV ? vr[(4 * ks + kq) * S::RS + ec] : s_sr[v][(4 * (ks - KSV) + kq) * S::RS + ec];
+        const float bf = ks < KSV ? vr[(4 * ks + kq) * S::RS + ec] : s_sr[v][(4 * (ks - KSV) + kq) * EB + ec];
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(afr[ks], bf, acc, 0, 0, 0);
         if (ks % 8 == 7) __builtin_amdgcn_sched_barrier(0);
       }
