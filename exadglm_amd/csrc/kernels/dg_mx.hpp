@@ -505,6 +505,8 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
 #pragma unroll 1
       for (int v = 0; v < 4; v++) {
         mx_volume_pass<N>(v + 1, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
+        if constexpr (RT == 6)  // the slot of contraction wave w - 2 (see below)
+          mx_volume_pass<N>(v + 1, w - 2, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
         if (v == 0) fetch_conn(e1, ne1);
         mx_barrier();
       }
@@ -602,7 +604,10 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
             }
         }
       }
-      if (v < 4)
+      // (at RT = 6, N = 6, contraction waves 4 and 5 share SIMDs 0 and 1 with
+      // waves 0 and 1, whose chains take the MFMA pipe first: the helpers, on
+      // SIMDs 2 and 3, run their volume-pass slots in the variable loop)
+      if (v < 4 && (RT != 6 || w < 4))
         mx_volume_pass<N>(v + 1, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[v + 1], s_vr[(v + 1) & 1]);
       mx_barrier();  // remainder tile of v + 1 complete; the tile of v is free
     }
