@@ -478,8 +478,13 @@ __global__ __launch_bounds__(MxShape<N>::NT) void stage_mx_kernel(StageArgs a, c
         }
       }
     }
-    // ---- 3. variable 0's volume pass ----
-    mx_volume_pass<N>(0, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[0], s_vr[0]);
+    // ---- 3. variable 0's volume pass, all eight slots on waves 0..3: the
+    //      waves of faces 2 and 3 (4..7), whose exterior traces lie farther
+    //      away in memory, finish the surface pass later ----
+    if (w < 4) {
+      mx_volume_pass<N>(0, w, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[0], s_vr[0]);
+      mx_volume_pass<N>(0, w + 4, lane, ne, &s_u[0][0], s_geo, pv, vv, gamma, s_c[0], s_vr[0]);
+    }
     mx_barrier();
   };
 
