@@ -17,23 +17,27 @@
 // and update, pressure, wave speed, fluxes, fits and the (A V) c term stay
 // FP64.
 //
-// Organisation: one persistent workgroup per CU of eight waves: RT =
-// ceil(Np/16) contraction waves and 8 - RT helpers.  Contraction wave t keeps
-// the FP32 A-fragments of rows 16t..16t+15 (all k-steps) and its rows of
-// (A V) in registers for the whole kernel.  Per 16-element batch (the 16 MFMA
-// columns), with the node values of the batch staged in LDS:
-//   1. node values, geometry, connectivity and tau into LDS;
+// Organisation: one persistent workgroup per CU of eight waves (wave w runs
+// on SIMD w % 4): RT = ceil(Np/16) contraction waves and 8 - RT helpers.
+// Contraction wave t keeps the FP32 A-fragments of rows 16t..16t+15 (all
+// k-steps) and its rows of (A V) in registers for the whole kernel.  Per
+// 16-element batch (the 16 MFMA columns):
+//   1. the helpers write the batch's node values, geometry and tau into LDS
+//      (they loaded them into registers during the previous batch's variable
+//      loop);
 //   2. the surface flux of all five variables at every face-node pair, once,
 //      its face fits, and the remainders rounded into an FP32 tile per
-//      variable (eight threads per face);
-//   3. variable 0's volume pass: volume fluxes, their fits, the remainder tile;
+//      variable (eight threads per face; waves 0..3 take faces 0 and 1,
+//      waves 4..7 faces 2 and 3, whose exterior traces lie farther away);
+//   3. variable 0's volume pass (volume fluxes, their fits, the remainder
+//      tile), on waves 0..3, which finish step 2 first;
 //   4. per variable v: the contraction waves run the MFMA contraction over
 //      [volume | surface] k-steps, the fit term (A V) c on FP64 MFMA and the
-//      LSRK update; then every wave takes two elements of the volume pass of
-//      variable v + 1 into the other remainder buffer; one barrier per
-//      variable.
-// Every wave takes part in the VALU passes (two elements of a volume pass
-// each), so their latency is spread over eight waves.
+//      LSRK update, while the volume pass of variable v + 1 goes into the
+//      other remainder buffer; one barrier per variable.
+// A volume pass is eight slots of two elements each.  In step 4 wave w takes
+// slot w, except at RT = 6 (N = 6), where contraction waves 4 and 5 share
+// SIMDs 0 and 1 with waves 0 and 1 and their slots go to the helpers.
 //
 // No Dirichlet faces and no advection here: those take the generic MFMA
 // kernel (dg_mfma.hpp).
