@@ -152,37 +152,14 @@ __device__ inline void mx_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_ba
 // Surface flux of the five variables times Fscale at one face-node pair:
 // LM (the reference's, euler_dglm.cpp:110-183) or Rusanov.  g = (nx, ny, nz,
 // Fscale) of the face.
-// Reciprocal and square root on the surface and volume passes' dependency
-// chains (the critical paths of the stage-in and of the variable loop): the
-// hardware estimates v_rcp_f64 and v_rsq_f64 refined by Newton steps (two
-// r += r (1 - x r); for the root a coupled step on g = x y, h = y / 2 and a
-// correction g += h (x - g^2)), within an ulp of the IEEE results and equal
-// to them on every one of 4M sampled densities in [1e-3, 1e3] and squared
-// sound speeds in [1e-6, 1e6]; the IEEE division and square-root sequences
-// are chains about twice as long.  For positive normal arguments (densities,
-// squared sound speeds); zero and NaN pass through the root.
-__device__ inline double mx_rcp(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = __builtin_fma(-x, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-x, r, 1.0);
-  return __builtin_fma(r, e, r);
-}
-__device__ inline double mx_sqrt(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y, h = 0.5 * y;
-  const double r = __builtin_fma(-h, g, 0.5);
-  g = __builtin_fma(g, r, g);
-  h = __builtin_fma(h, r, h);
-  g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
-  return x > 0.0 ? g : x;
-}
-// euler_normal_flux5 / euler_rusanov5 (dg_fp64.hpp) with those two
+// euler_normal_flux5 / euler_rusanov5 (dg_fp64.hpp) with rcp_refined and
+// sqrt_refined: the surface and volume passes are the critical paths of the
+// stage-in and the variable loop
 __device__ inline void mx_normal_flux5(const double U[5], const double *n, double gamma, double F[5], double &lam) {
-  const double inv = mx_rcp(U[0]);
+  const double inv = rcp_refined(U[0]);
   const double p = (gamma - 1.0) * (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) * inv);
   const double vn = (n[0] * U[1] + n[1] * U[2] + n[2] * U[3]) * inv;
-  lam = fabs(vn) + mx_sqrt(fabs(gamma * p * inv));
+  lam = fabs(vn) + sqrt_refined(fabs(gamma * p * inv));
   F[0] = vn * U[0];
   F[1] = U[1] * vn + n[0] * p;
   F[2] = U[2] * vn + n[1] * p;
@@ -206,7 +183,7 @@ __device__ inline void mx_surface_flux5(const double uM[5], const double uP[5], 
   double ub[5];
 #pragma unroll
   for (int v = 0; v < 5; v++) ub[v] = 0.5 * (uM[v] + uP[v]);
-  const double inv = mx_rcp(ub[0]);
+  const double inv = rcp_refined(ub[0]);
   const double p = (gamma - 1.0) * (ub[4] - 0.5 * (ub[1] * ub[1] + ub[2] * ub[2] + ub[3] * ub[3]) * inv);
   const double vn = (g[0] * ub[1] + g[1] * ub[2] + g[2] * ub[3]) * inv;
   double nF[5];
@@ -256,7 +233,7 @@ __device__ inline double sum32(double x) {
 // here goes through private memory).
 __device__ inline void mx_volume_flux1(int v, const double u[5], const double *gm, double gamma, double f[3]) {
   const double r = u[0], mx = u[1], my = u[2], mz = u[3], E = u[4];
-  const double inv = mx_rcp(r);
+  const double inv = rcp_refined(r);
   const double p = (gamma - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) * inv);
   const double vx = mx * inv, vy = my * inv, vz = mz * inv;
   const double mv = v == 1 ? mx : (v == 2 ? my : mz);  // momentum component of v = 1..3
