@@ -16,7 +16,7 @@ run() reads the datain/ files just written (the same file contract), sets the
 reference initial condition (advec_init.cpp / euler_init.cpp) unless an
 initial state was given with set_initial_state(), and writes dataout/ at the
 reference cadence (find_print_tstep, euler_time.cpp:4-12).  Both the CPU and
-the GPU classes run on the GPU: this build has no CPU solver.
+the GPU classes run on the GPU: the library has no CPU solver.
 """
 import json
 import os
